@@ -1,0 +1,1091 @@
+// n4_capi.hip -- host driver behind the C ABI (include/n4_capi.h).
+//
+// A call plans the batch on the host (working grids, per-level control
+// point counts, FP64 axis tables, slab lists, arena offsets), uploads the
+// plan with one copy, then enqueues the level loop on the caller's stream:
+//   prep -> for each level: level setup, fit(DEN), iterations x
+//   {hist(+Wiener tail), fit(NUM, + lattice tail), eval(+CV tail)} -> recon.
+// Nothing is allocated inside the loop: the handle's arena grows only when a
+// larger batch arrives.  Under N4_EMULATE the same driver runs the kernel
+// bodies on the host (CPU unit tests of the kernel logic only).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/n4_capi.h"
+#include "n4_kernels.h"
+
+#ifndef N4_EMULATE
+#include <hip/hip_runtime.h>
+#endif
+
+using namespace n4;
+
+// ----------------------------------------------------------- error state --
+static thread_local std::string g_err;
+
+static n4_status fail(n4_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+// --------------------------------------------------- device abstraction --
+#ifndef N4_EMULATE
+#define N4_CHECK_HIP(expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(N4_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));         \
+    } while (0)
+static int dev_alloc(void** p, size_t n) { return hipMalloc(p, n) == hipSuccess ? 0 : 1; }
+static void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+#define H2D(dst, src, n, st) N4_CHECK_HIP(hipMemcpyAsync((dst), (src), (n), hipMemcpyHostToDevice, (hipStream_t)(st)))
+#define D2H(dst, src, n, st) N4_CHECK_HIP(hipMemcpyAsync((dst), (src), (n), hipMemcpyDeviceToHost, (hipStream_t)(st)))
+#define D2D(dst, src, n, st) N4_CHECK_HIP(hipMemcpyAsync((dst), (src), (n), hipMemcpyDeviceToDevice, (hipStream_t)(st)))
+#define SYNC(st) N4_CHECK_HIP(hipStreamSynchronize((hipStream_t)(st)))
+#define LAUNCH(expr)                                                                         \
+    do {                                                                                     \
+        int e_ = (expr);                                                                     \
+        if (e_ != 0)                                                                         \
+            return fail(N4_ERR_HIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)e_)); \
+    } while (0)
+#else
+static int dev_alloc(void** p, size_t n) {
+    *p = aligned_alloc(256, (n + 255) & ~size_t(255));
+    if (*p) memset(*p, 0, n);
+    return *p ? 0 : 1;
+}
+static void dev_free(void* p) { free(p); }
+#define H2D(dst, src, n, st) memcpy((dst), (src), (n))
+#define D2H(dst, src, n, st) memcpy((dst), (src), (n))
+#define D2D(dst, src, n, st) memcpy((dst), (src), (n))
+#define SYNC(st) \
+    do {         \
+    } while (0)
+#define LAUNCH(expr) (void)(expr)
+#endif
+
+// ---------------------------------------------------------------- handle --
+static const int kPipelineSlots = 4;
+
+struct n4_handle {
+    int device = 0;
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<unsigned char> staging;  // host image of the upload region
+    int32_t* pinned_flags = nullptr;     // flag_cap level-tagged done flags (pinned host)
+    int32_t* dev_flags = nullptr;        // the same memory as seen by the device
+    int flag_cap = 0;
+#ifndef N4_EMULATE
+    hipEvent_t flag_ev[kPipelineSlots];
+    bool flag_ev_init = false;
+    std::vector<hipEvent_t> prof_ev;       // event pool for kernel timing
+    std::vector<int> prof_pending;         // kernel id per recorded pair
+#endif
+    uint32_t prof_mask = 0;
+    double prof_ms[N4_K_COUNT] = {0};
+    uint64_t prof_n[N4_K_COUNT] = {0};
+};
+
+// Early exit: the convergence tail writes each volume's done flag, tagged
+// with level+1, straight into pinned host memory; the host records an event
+// per iteration and, kPipeline iterations behind the GPU, stops enqueueing
+// iterations once every volume is done at the current level.
+static const int kPipeline = 3;
+static const int kFlagSlots = kPipelineSlots;
+
+static n4_status ensure_arena(n4_handle* h, size_t bytes) {
+    if (bytes <= h->arena_bytes) return N4_OK;
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+    N4_CHECK_HIP(hipDeviceSynchronize());
+#endif
+    dev_free(h->arena);
+    h->arena = nullptr;
+    h->arena_bytes = 0;
+    size_t want = bytes + bytes / 8 + (1u << 20);
+    if (dev_alloc(&h->arena, want))
+        return fail(N4_ERR_NO_MEMORY, "device allocation of %zu bytes failed", want);
+    h->arena_bytes = want;
+    return N4_OK;
+}
+
+// ---------------------------------------------------------------- tables --
+static void cubic_bspline(double t, double w[4]) {  // N4.py:174-181
+    const double t2 = t * t, t3 = t * t * t, om = 1.0 - t;
+    w[0] = om * om * om / 6.0;
+    w[1] = (3.0 * t3 - 6.0 * t2 + 4.0) / 6.0;
+    w[2] = (-3.0 * t3 + 3.0 * t2 + 3.0 * t + 1.0) / 6.0;
+    w[3] = t3 / 6.0;
+}
+
+// Span index / weights per position (N4.py:192, 210-217) and, per control
+// point, the contiguous range of positions whose 4-support contains it.
+static void axis_table(int npos, double domain, int ncpt, AxisEntry* out, int32_t* sup) {
+    const double span = domain / (double)(ncpt - 3);
+    for (int c = 0; c < ncpt; ++c) {
+        sup[2 * c] = npos;
+        sup[2 * c + 1] = 0;
+    }
+    for (int i = 0; i < npos; ++i) {
+        const double pos = (double)i + 0.5;
+        int b = (int)floor(pos / span);
+        const double t = (pos - (double)b * span) / span;
+        if (b < 0) b = 0;
+        if (b > ncpt - 4) b = ncpt - 4;
+        double w[4];
+        cubic_bspline(t, w);
+        const double S = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+        AxisEntry& e = out[i];
+        memset(&e, 0, sizeof e);
+        for (int a = 0; a < 4; ++a) {
+            e.w[a] = (float)w[a];
+            e.p[a] = (float)(w[a] * w[a] * w[a] / S);
+            e.q[a] = (float)(w[a] * w[a]);
+            const int c = b + a;
+            if (i < sup[2 * c]) sup[2 * c] = i;
+            if (i + 1 > sup[2 * c + 1]) sup[2 * c + 1] = i + 1;
+        }
+        e.b = b;
+    }
+    for (int c = 0; c < ncpt; ++c)
+        if (sup[2 * c] >= sup[2 * c + 1]) sup[2 * c] = sup[2 * c + 1] = 0;
+}
+
+static int padded_size(int nbins, int* log2P) {  // N4.py:31-37
+    int e = 0;
+    while ((1 << e) < nbins) ++e;
+    *log2P = e + 1;
+    return 1 << (e + 1);
+}
+
+static RunParams make_run_params(const n4_params* p) {
+    RunParams rp;
+    memset(&rp, 0, sizeof rp);
+    rp.nbins = (int)p->number_of_histogram_bins;
+    rp.P = padded_size(rp.nbins, &rp.log2P);
+    rp.offset = (int)(0.5 * (double)(rp.P - rp.nbins));
+    rp.noise = p->wiener_filter_noise;
+    rp.fwhm = p->bias_field_fwhm;
+    rp.thr = p->convergence_threshold;
+    rp.eps = p->fit_eps;
+    rp.low = p->low_value;
+    float f = (float)p->low_value;
+    if ((double)f > p->low_value) f = nextafterf(f, -INFINITY);
+    rp.low_f = f;
+    rp.fixed_iters = (p->flags & N4_FLAG_FIXED_ITERATIONS) ? 1 : 0;
+    return rp;
+}
+
+// ----------------------------------------------------------------- plans --
+// Bump layout of one arena: offsets first, pointers after allocation.
+struct Layout {
+    size_t off = 0;
+    size_t take(size_t bytes, size_t align = 256) {
+        off = (off + align - 1) / align * align;
+        size_t o = off;
+        off += bytes;
+        return o;
+    }
+};
+
+struct VolIn {  // one volume of a request
+    const float* im;
+    const void* mask;
+    int mask_kind;
+    int F[3];
+    int s;
+    float* out_norm;
+    float* out_bias;
+    float* out_logbias;
+    float* out_lattice;
+};
+
+struct Plan {
+    int nvol = 0;
+    int nlevels = 0;
+    RunParams rp;
+    std::vector<VolDesc> desc;  // host copies (pointers into the arena)
+    size_t upload_bytes = 0;    // [0, upload_bytes) of the arena comes from staging
+    size_t total_bytes = 0;
+    size_t off_desc = 0, off_wslab = 0, off_fslab = 0, off_state = 0;
+    int n_wslab = 0, n_fslab = 0;
+    std::vector<int> max_iter;  // per level (max over volumes)
+    size_t lds_fit_max[kMaxLevels] = {0}, lds_eval_max[kMaxLevels] = {0};
+    size_t lds_hist = 0, lds_recon = 0;
+};
+
+static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
+
+// Builds the plan and fills h->staging with the upload region.  Pointers in
+// the descriptors are arena-relative until `rebase` adds the arena base.
+static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, Plan* P,
+                            bool need_tables_full = true) {
+    const int nv = (int)in.size();
+    const int L = (int)p->number_of_fitting_levels;
+    P->nvol = nv;
+    P->nlevels = L;
+    P->rp = make_run_params(p);
+    P->max_iter.assign(L, 0);
+    for (int l = 0; l < L; ++l) P->max_iter[l] = (int)p->max_iterations[l];
+
+    // ---- sizes
+    struct VInfo {
+        int W[3];
+        int C[kMaxLevels][3];
+        size_t Lmax, part_stride;
+    };
+    std::vector<VInfo> vi(nv);
+    int nws = 0, nfs = 0;
+    for (int v = 0; v < nv; ++v) {
+        VInfo& I = vi[v];
+        for (int d = 0; d < 3; ++d) I.W[d] = (in[v].F[d] + in[v].s - 1) / in[v].s;
+        for (int l = 0; l < L; ++l)
+            for (int d = 0; d < 3; ++d) I.C[l][d] = level_ncpt((int)p->ncpt[d], l);
+        const int* Cf = I.C[L - 1];
+        I.Lmax = (size_t)Cf[0] * Cf[1] * Cf[2];
+        I.part_stride = (size_t)Cf[1] * Cf[2];
+        nws += I.W[0];
+        nfs += in[v].F[0];
+    }
+    P->n_wslab = nws;
+    P->n_fslab = need_tables_full ? nfs : 0;
+
+    Layout lay;
+    P->off_desc = lay.take(sizeof(VolDesc) * nv);
+    P->off_wslab = lay.take(sizeof(Slab) * (nws > 0 ? nws : 1));
+    P->off_fslab = lay.take(sizeof(Slab) * (P->n_fslab > 0 ? P->n_fslab : 1));
+    P->off_state = lay.take(sizeof(VolState) * nv);
+    // tables
+    std::vector<size_t> off_ax(nv * kMaxLevels * 3), off_sup(nv * kMaxLevels * 3), off_fax(nv * 3);
+    for (int v = 0; v < nv; ++v)
+        for (int l = 0; l < L; ++l)
+            for (int d = 0; d < 3; ++d) {
+                off_ax[(v * kMaxLevels + l) * 3 + d] = lay.take(sizeof(AxisEntry) * vi[v].W[d], 64);
+                off_sup[(v * kMaxLevels + l) * 3 + d] = lay.take(sizeof(int32_t) * 2 * vi[v].C[l][d], 16);
+            }
+    if (need_tables_full)
+        for (int v = 0; v < nv; ++v)
+            for (int d = 0; d < 3; ++d) off_fax[v * 3 + d] = lay.take(sizeof(AxisEntry) * in[v].F[d], 64);
+    P->upload_bytes = lay.take(0);
+
+    // per-volume working buffers
+    struct Offs {
+        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, hpart, spart, etab;
+    };
+    std::vector<Offs> off(nv);
+    const int nb = P->rp.nbins;
+    for (int v = 0; v < nv; ++v) {
+        const VInfo& I = vi[v];
+        const size_t nw = (size_t)I.W[0] * I.W[1] * I.W[2];
+        Offs& o = off[v];
+        o.imlog = lay.take(sizeof(float) * nw);
+        o.logU = lay.take(sizeof(float) * nw);
+        o.wmask = lay.take(nw);
+        o.lat0 = lay.take(sizeof(double) * I.Lmax);
+        o.lat1 = lay.take(sizeof(double) * I.Lmax);
+        o.den = lay.take(sizeof(double) * I.Lmax);
+        o.latf = lay.take(sizeof(float) * I.Lmax);
+        o.lresf = lay.take(sizeof(float) * I.Lmax);
+        o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
+        o.hpart = lay.take(sizeof(uint64_t) * I.W[0] * nb);
+        o.spart = lay.take(sizeof(double) * 4 * I.W[0]);
+        o.etab = lay.take(sizeof(double) * nb);
+    }
+    P->total_bytes = lay.take(0);
+
+    n4_status s = ensure_arena(h, P->total_bytes);
+    if (s != N4_OK) return s;
+    unsigned char* base = (unsigned char*)h->arena;
+
+    // ---- host image of the upload region
+    h->staging.assign(P->upload_bytes, 0);
+    unsigned char* hb = h->staging.data();
+    P->desc.assign(nv, VolDesc());
+    Slab* ws = (Slab*)(hb + P->off_wslab);
+    Slab* fs = (Slab*)(hb + P->off_fslab);
+    int iws = 0, ifs = 0;
+    for (int v = 0; v < nv; ++v) {
+        const VInfo& I = vi[v];
+        VolDesc D;
+        memset(&D, 0, sizeof D);
+        D.im = in[v].im;
+        D.mask = in[v].mask;
+        D.mask_kind = in[v].mask_kind;
+        D.s = in[v].s;
+        D.out_norm = in[v].out_norm;
+        D.out_bias = in[v].out_bias;
+        D.out_logbias = in[v].out_logbias;
+        D.out_lattice = in[v].out_lattice;
+        for (int d = 0; d < 3; ++d) {
+            D.F[d] = in[v].F[d];
+            D.W[d] = I.W[d];
+        }
+        D.nvox_w = (int64_t)I.W[0] * I.W[1] * I.W[2];
+        const Offs& o = off[v];
+        D.imlog = (float*)(base + o.imlog);
+        D.logU = (float*)(base + o.logU);
+        D.wmask = (uint8_t*)(base + o.wmask);
+        D.lat[0] = (double*)(base + o.lat0);
+        D.lat[1] = (double*)(base + o.lat1);
+        D.den = (double*)(base + o.den);
+        D.latf = (float*)(base + o.latf);
+        D.lresf = (float*)(base + o.lresf);
+        D.part = (float*)(base + o.part);
+        D.hpart = (uint64_t*)(base + o.hpart);
+        D.spart = (double*)(base + o.spart);
+        D.etab = (double*)(base + o.etab);
+        D.st = (VolState*)(base + P->off_state) + v;
+        D.done_flag = (h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
+        D.nlevels = L;
+        D.part_stride = (int32_t)I.part_stride;
+        for (int l = 0; l < L; ++l) {
+            D.max_iter[l] = (int)p->max_iterations[l];
+            LevelDesc& LD = D.lev[l];
+            for (int d = 0; d < 3; ++d) {
+                LD.C[d] = I.C[l][d];
+                const size_t oa = off_ax[(v * kMaxLevels + l) * 3 + d];
+                const size_t os = off_sup[(v * kMaxLevels + l) * 3 + d];
+                axis_table(I.W[d], (double)I.W[d], I.C[l][d], (AxisEntry*)(hb + oa), (int32_t*)(hb + os));
+                LD.ax[d] = (const AxisEntry*)(base + oa);
+                LD.sup[d] = (const int32_t*)(base + os);
+            }
+            LD.L = LD.C[0] * LD.C[1] * LD.C[2];
+        }
+        if (need_tables_full) {
+            const int* Cf = I.C[L - 1];
+            for (int d = 0; d < 3; ++d) {
+                const size_t oa = off_fax[v * 3 + d];
+                std::vector<int32_t> sup(2 * Cf[d]);
+                axis_table(in[v].F[d], (double)I.W[d] * (double)in[v].s, Cf[d], (AxisEntry*)(hb + oa), sup.data());
+                D.fax[d] = (const AxisEntry*)(base + oa);
+            }
+        }
+        P->desc[v] = D;
+        for (int x = 0; x < I.W[0]; ++x) ws[iws++] = Slab{v, x};
+        if (need_tables_full)
+            for (int x = 0; x < in[v].F[0]; ++x) fs[ifs++] = Slab{v, x};
+        for (int l = 0; l < L; ++l) {
+            P->lds_fit_max[l] = std::max(P->lds_fit_max[l], lds_fit(nb, D, l));
+            P->lds_eval_max[l] = std::max(P->lds_eval_max[l], lds_eval(D, l));
+        }
+        if (need_tables_full) P->lds_recon = std::max(P->lds_recon, lds_recon(D));
+    }
+    P->lds_hist = lds_hist(nb, P->rp.P);
+    memcpy(hb + P->off_desc, P->desc.data(), sizeof(VolDesc) * nv);
+    // VolState region stays zero: counters start at 0.
+    return N4_OK;
+}
+
+// LDS of every kernel a grid plan launches must fit one workgroup.
+static const size_t kLdsCap = 64 * 1024;
+static n4_status check_lds(const Plan& P, bool grid_kernels, bool recon) {
+    size_t worst = P.lds_hist;
+    if (grid_kernels)
+        for (int l = 0; l < P.nlevels; ++l) worst = std::max(worst, std::max(P.lds_fit_max[l], P.lds_eval_max[l]));
+    if (recon) worst = std::max(worst, P.lds_recon);
+    if (worst > kLdsCap)
+        return fail(N4_ERR_UNSUPPORTED, "volume/lattice too large for one workgroup's LDS (%zu bytes)", worst);
+    return N4_OK;
+}
+
+static inline VolDesc* dev_desc(n4_handle* h, const Plan& P) {
+    return (VolDesc*)((unsigned char*)h->arena + P.off_desc);
+}
+static inline Slab* dev_wslab(n4_handle* h, const Plan& P) {
+    return (Slab*)((unsigned char*)h->arena + P.off_wslab);
+}
+static inline Slab* dev_fslab(n4_handle* h, const Plan& P) {
+    return (Slab*)((unsigned char*)h->arena + P.off_fslab);
+}
+static inline VolState* dev_state(n4_handle* h, const Plan& P) {
+    return (VolState*)((unsigned char*)h->arena + P.off_state);
+}
+
+// --------------------------------------------------------------- checks --
+static n4_status check_params(const n4_params* p) {
+    if (!p) return fail(N4_ERR_INVALID_ARG, "params is NULL");
+    if (p->number_of_histogram_bins < 2 || p->number_of_histogram_bins > N4_MAX_BINS ||
+        p->number_of_histogram_bins > (uint32_t)kMaxBins)
+        return fail(N4_ERR_INVALID_ARG, "number_of_histogram_bins must be in [2, %d]", kMaxBins);
+    if (p->number_of_fitting_levels < 1 || p->number_of_fitting_levels > N4_MAX_LEVELS)
+        return fail(N4_ERR_INVALID_ARG, "number_of_fitting_levels must be in [1, %d]", N4_MAX_LEVELS);
+    for (int d = 0; d < 3; ++d)
+        if (p->ncpt[d] < 4 || p->ncpt[d] > 4096)
+            return fail(N4_ERR_INVALID_ARG, "ncpt[%d] = %u: need >= 4 control points (spline order 3)", d,
+                        p->ncpt[d]);
+    for (uint32_t l = 0; l < p->number_of_fitting_levels; ++l)
+        if (p->max_iterations[l] > 100000) return fail(N4_ERR_INVALID_ARG, "max_iterations[%u] too large", l);
+    if (!(p->bias_field_fwhm > 0.0)) return fail(N4_ERR_INVALID_ARG, "bias_field_fwhm must be > 0");
+    if (!(p->wiener_filter_noise >= 0.0)) return fail(N4_ERR_INVALID_ARG, "wiener_filter_noise must be >= 0");
+    if (!(p->fit_eps >= 0.0)) return fail(N4_ERR_INVALID_ARG, "fit_eps must be >= 0");
+    if (p->shrink_factor < 1 || p->shrink_factor > 64) return fail(N4_ERR_INVALID_ARG, "shrink_factor must be in [1, 64]");
+    if (p->precision != N4_PREC_FP32)
+        return fail(N4_ERR_UNSUPPORTED, "precision %u: only N4_PREC_FP32 is implemented in this build",
+                    p->precision);
+    // final lattice size must stay sane
+    uint64_t Lf = 1;
+    for (int d = 0; d < 3; ++d) {
+        uint64_t c = (uint64_t)(p->ncpt[d] - 3) << (p->number_of_fitting_levels - 1);
+        if (c > 4096) return fail(N4_ERR_INVALID_ARG, "final lattice too large on axis %d", d);
+        Lf *= c + 3;
+    }
+    if (Lf > (1u << 24)) return fail(N4_ERR_INVALID_ARG, "final lattice too large");
+    return N4_OK;
+}
+
+static n4_status check_volume(const VolIn& v) {
+    for (int d = 0; d < 3; ++d)
+        if (v.F[d] < 1 || v.F[d] > 65536) return fail(N4_ERR_INVALID_ARG, "dimension %d = %d out of range", d, v.F[d]);
+    if ((uint64_t)v.F[0] * v.F[1] * v.F[2] > (1ull << 34)) return fail(N4_ERR_INVALID_ARG, "volume too large");
+    if (!v.im) return fail(N4_ERR_INVALID_ARG, "input image pointer is NULL");
+    if (!v.out_norm) return fail(N4_ERR_INVALID_ARG, "normalized output pointer is NULL");
+    if (v.mask_kind < 0 || v.mask_kind > 2) return fail(N4_ERR_INVALID_ARG, "mask_kind %d invalid", v.mask_kind);
+    if (v.mask_kind != N4_MASK_NONE && !v.mask) return fail(N4_ERR_INVALID_ARG, "mask pointer is NULL");
+    if (v.s < 1 || v.s > 64) return fail(N4_ERR_INVALID_ARG, "shrink factor %d out of range", v.s);
+    return N4_OK;
+}
+
+// --------------------------------------------------------------- engine --
+// Brackets one launch with an event pair when its kernel is profiled.
+struct ProfScope {
+    n4_handle* h;
+    int id;
+    void* stream;
+    bool on;
+    ProfScope(n4_handle* h_, int id_, void* s_) : h(h_), id(id_), stream(s_), on((h_->prof_mask >> id_) & 1u) {
+#ifndef N4_EMULATE
+        if (on) record();
+#endif
+    }
+    ~ProfScope() {
+#ifndef N4_EMULATE
+        if (on) {
+            record();
+            h->prof_pending.push_back(id);
+        }
+#endif
+    }
+#ifndef N4_EMULATE
+    void record() {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, (hipStream_t)stream);
+        h->prof_ev.push_back(e);
+    }
+#endif
+};
+
+// Folds recorded event pairs into the accumulators (after a stream sync).
+static void prof_collect(n4_handle* h) {
+#ifndef N4_EMULATE
+    const size_t npairs = std::min(h->prof_pending.size(), h->prof_ev.size() / 2);
+    for (size_t i = 0; i < npairs; ++i) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]) == hipSuccess) {
+            h->prof_ms[h->prof_pending[i]] += ms;
+            h->prof_n[h->prof_pending[i]] += 1;
+        }
+    }
+    for (hipEvent_t e : h->prof_ev) (void)hipEventDestroy(e);
+    h->prof_ev.clear();
+    h->prof_pending.clear();
+#else
+    (void)h;
+#endif
+}
+
+#define TIMED(id, expr)                          \
+    do {                                         \
+        ProfScope ps_(h, (id), stream);          \
+        LAUNCH(expr);                            \
+    } while (0)
+
+static n4_status ensure_flags(n4_handle* h, int nvol) {
+    if (nvol <= h->flag_cap) return N4_OK;
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipDeviceSynchronize());
+    if (h->pinned_flags) (void)hipHostFree(h->pinned_flags);
+    h->pinned_flags = nullptr;
+    N4_CHECK_HIP(hipHostMalloc((void**)&h->pinned_flags, sizeof(int32_t) * nvol, hipHostMallocDefault));
+    N4_CHECK_HIP(hipHostGetDevicePointer((void**)&h->dev_flags, h->pinned_flags, 0));
+    if (!h->flag_ev_init) {
+        for (int k = 0; k < kFlagSlots; ++k) N4_CHECK_HIP(hipEventCreateWithFlags(&h->flag_ev[k], hipEventDisableTiming));
+        h->flag_ev_init = true;
+    }
+#else
+    free(h->pinned_flags);
+    h->pinned_flags = (int32_t*)calloc((size_t)nvol, sizeof(int32_t));
+    h->dev_flags = h->pinned_flags;
+#endif
+    h->flag_cap = nvol;
+    return N4_OK;
+}
+
+// Records the end of iteration `it` and inspects the flags once iteration
+// it - kPipeline has completed.  Flags only move from 0 to level+1 within a
+// level, so reading a newer state than that iteration's is harmless.
+static n4_status poll_done(n4_handle* h, int nvol, int level, int it, void* stream, bool* all_done) {
+    *all_done = false;
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipEventRecord(h->flag_ev[it % kFlagSlots], (hipStream_t)stream));
+    const int old = it - kPipeline;
+    if (old < 0) return N4_OK;
+    N4_CHECK_HIP(hipEventSynchronize(h->flag_ev[old % kFlagSlots]));
+#else
+    (void)stream;
+    (void)it;
+#endif
+    const volatile int32_t* f = h->pinned_flags;
+    bool all = true;
+    for (int v = 0; v < nvol && all; ++v) all = f[v] == level + 1;
+    *all_done = all;
+    return N4_OK;
+}
+
+static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, n4_report* reports,
+                           void* stream) {
+    g_err.clear();
+    n4_status s = check_params(p);
+    if (s != N4_OK) return s;
+    for (const VolIn& v : in) {
+        s = check_volume(v);
+        if (s != N4_OK) return s;
+    }
+    if (in.empty()) return N4_OK;
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+    hipEvent_t e0, e1;
+    N4_CHECK_HIP(hipEventCreate(&e0));
+    N4_CHECK_HIP(hipEventCreate(&e1));
+    N4_CHECK_HIP(hipEventRecord(e0, (hipStream_t)stream));
+#endif
+    s = ensure_flags(h, (int)in.size());
+    if (s != N4_OK) return s;
+    for (size_t v = 0; v < in.size(); ++v) h->pinned_flags[v] = 0;
+    Plan P;
+    s = build_plan(h, in, p, &P);
+    if (s != N4_OK) return s;
+    s = check_lds(P, true, true);
+    if (s != N4_OK) return s;
+    H2D(h->arena, h->staging.data(), P.upload_bytes, stream);
+    VolDesc* dd = dev_desc(h, P);
+    Slab* wsl = dev_wslab(h, P);
+    Slab* fsl = dev_fslab(h, P);
+    const RunParams rp = P.rp;
+    TIMED(N4_K_PREP, launch_prep(dd, wsl, P.n_wslab, rp, stream));
+    for (int l = 0; l < P.nlevels; ++l) {
+        TIMED(N4_K_LEVEL, launch_level(dd, P.nvol, l, rp, stream));
+        TIMED(N4_K_FIT_DEN, launch_fit(dd, wsl, P.n_wslab, l, kFitDen, rp, P.lds_fit_max[l], stream));
+        for (int it = 0; it < P.max_iter[l]; ++it) {
+            TIMED(N4_K_HIST, launch_hist(dd, wsl, P.n_wslab, rp, nullptr, P.lds_hist, stream));
+            TIMED(N4_K_FIT, launch_fit(dd, wsl, P.n_wslab, l, kFitNum, rp, P.lds_fit_max[l], stream));
+            TIMED(N4_K_EVAL, launch_eval(dd, wsl, P.n_wslab, l, rp, P.lds_eval_max[l], stream));
+            if (it + 1 == P.max_iter[l]) break;
+            bool all_done = false;
+            s = poll_done(h, P.nvol, l, it, stream, &all_done);
+            if (s != N4_OK) return s;
+            if (all_done) break;
+        }
+    }
+    TIMED(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, stream));
+    for (int v = 0; v < P.nvol; ++v)
+        if (in[v].out_lattice) {
+            const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
+            D2D(in[v].out_lattice, P.desc[v].latf, sizeof(float) * LD.L, stream);
+        }
+    std::vector<VolState> st(P.nvol);
+    D2H(st.data(), dev_state(h, P), sizeof(VolState) * P.nvol, stream);
+    float ms = 0.0f;
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipEventRecord(e1, (hipStream_t)stream));
+    N4_CHECK_HIP(hipEventSynchronize(e1));
+    N4_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    N4_CHECK_HIP(hipGetLastError());
+#endif
+    prof_collect(h);
+    if (reports)
+        for (int v = 0; v < P.nvol; ++v) {
+            n4_report& r = reports[v];
+            memset(&r, 0, sizeof r);
+            for (int l = 0; l < P.nlevels; ++l) {
+                r.iterations[l] = (uint32_t)st[v].iters[l];
+                r.last_convergence[l] = st[v].lastcv[l];
+            }
+            r.n_mask_full = st[v].n_mask_full;
+            r.n_mask_work = st[v].n_mask_work;
+            for (int d = 0; d < 3; ++d) {
+                r.work_dims[d] = (uint32_t)P.desc[v].W[d];
+                r.final_ncpt[d] = (uint32_t)P.desc[v].lev[P.nlevels - 1].C[d];
+            }
+            r.elapsed_ms = ms;
+        }
+    return N4_OK;
+}
+
+// ================================================================ C ABI ==
+extern "C" {
+
+const char* n4_version(void) {
+#ifdef N4_EMULATE
+    return "n4amd 0.1.0 (host emulation of the kernel bodies; test use only)";
+#else
+    return "n4amd 0.1.0 (gfx950)";
+#endif
+}
+
+const char* n4_status_string(n4_status s) {
+    switch (s) {
+        case N4_OK: return "ok";
+        case N4_ERR_INVALID_ARG: return "invalid argument";
+        case N4_ERR_HIP: return "HIP runtime error";
+        case N4_ERR_NO_MEMORY: return "out of device memory";
+        case N4_ERR_UNSUPPORTED: return "unsupported";
+        case N4_ERR_INTERNAL: return "internal error";
+    }
+    return "unknown status";
+}
+
+const char* n4_last_error(void) { return g_err.c_str(); }
+
+n4_status n4_params_default(n4_params* p) {
+    if (!p) return fail(N4_ERR_INVALID_ARG, "params is NULL");
+    memset(p, 0, sizeof *p);
+    p->number_of_histogram_bins = 200;
+    p->wiener_filter_noise = 0.01;
+    p->bias_field_fwhm = 0.15;
+    p->ncpt[0] = p->ncpt[1] = p->ncpt[2] = 4;
+    p->number_of_fitting_levels = 4;
+    for (int l = 0; l < N4_MAX_LEVELS; ++l) p->max_iterations[l] = 50;
+    p->convergence_threshold = 0.001;
+    p->low_value = 10.0;
+    p->shrink_factor = 1;
+    p->fit_eps = 1e-3;
+    p->precision = N4_PREC_FP32;
+    p->flags = 0;
+    return N4_OK;
+}
+
+n4_status n4_final_ncpt(const n4_params* p, uint32_t out[3]) {
+    if (!p || !out) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    if (p->number_of_fitting_levels < 1 || p->number_of_fitting_levels > N4_MAX_LEVELS)
+        return fail(N4_ERR_INVALID_ARG, "number_of_fitting_levels out of range");
+    for (int d = 0; d < 3; ++d) {
+        if (p->ncpt[d] < 4) return fail(N4_ERR_INVALID_ARG, "ncpt < 4");
+        out[d] = (uint32_t)level_ncpt((int)p->ncpt[d], (int)p->number_of_fitting_levels - 1);
+    }
+    return N4_OK;
+}
+
+n4_status n4_device_count(int* count) {
+    if (!count) return fail(N4_ERR_INVALID_ARG, "count is NULL");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipGetDeviceCount(count));
+#else
+    *count = 1;
+#endif
+    return N4_OK;
+}
+
+n4_status n4_create(n4_handle** out, int device) {
+    if (!out) return fail(N4_ERR_INVALID_ARG, "handle pointer is NULL");
+    *out = nullptr;
+#ifndef N4_EMULATE
+    int n = 0;
+    N4_CHECK_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(N4_ERR_INVALID_ARG, "device %d not present (%d devices)", device, n);
+    N4_CHECK_HIP(hipSetDevice(device));
+#endif
+    n4_handle* h = new n4_handle();
+    h->device = device;
+    *out = h;
+    return N4_OK;
+}
+
+n4_status n4_destroy(n4_handle* h) {
+    if (!h) return N4_OK;
+#ifndef N4_EMULATE
+    (void)hipSetDevice(h->device);
+    (void)hipDeviceSynchronize();
+#endif
+    dev_free(h->arena);
+#ifndef N4_EMULATE
+    if (h->pinned_flags) (void)hipHostFree(h->pinned_flags);
+    if (h->flag_ev_init)
+        for (int k = 0; k < kFlagSlots; ++k) (void)hipEventDestroy(h->flag_ev[k]);
+#else
+    free(h->pinned_flags);
+#endif
+    delete h;
+    return N4_OK;
+}
+
+n4_status n4_correct(n4_handle* h, const float* d_im, const void* d_mask, uint32_t mask_kind, uint32_t fx,
+                     uint32_t fy, uint32_t fz, const n4_params* p, float* d_im_normalized, float* d_lattice,
+                     float* d_bias_field, float* d_log_bias_field, n4_report* report, void* stream) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+    if (!p) return fail(N4_ERR_INVALID_ARG, "params is NULL");
+    VolIn v{d_im, d_mask, (int)mask_kind, {(int)fx, (int)fy, (int)fz}, (int)p->shrink_factor,
+            d_im_normalized, d_bias_field, d_log_bias_field, d_lattice};
+    return run_batch(h, std::vector<VolIn>{v}, p, report, stream);
+}
+
+n4_status n4_correct_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, const n4_params* p,
+                           n4_report* reports, void* stream) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+    if (!p) return fail(N4_ERR_INVALID_ARG, "params is NULL");
+    if (nvol > 0 && !vols) return fail(N4_ERR_INVALID_ARG, "volume list is NULL");
+    std::vector<VolIn> in;
+    in.reserve(nvol);
+    for (uint32_t i = 0; i < nvol; ++i) {
+        const n4_volume& a = vols[i];
+        VolIn v{a.d_im, a.d_mask, (int)a.mask_kind, {(int)a.dims[0], (int)a.dims[1], (int)a.dims[2]},
+                (int)(a.shrink_factor ? a.shrink_factor : p->shrink_factor), a.d_im_normalized, a.d_bias_field,
+                a.d_log_bias_field, a.d_lattice};
+        in.push_back(v);
+    }
+    return run_batch(h, in, p, reports, stream);
+}
+
+n4_status n4_correct_host(n4_handle* h, const float* h_im, const float* h_mask, uint32_t fx, uint32_t fy,
+                          uint32_t fz, const n4_params* p, float* h_im_normalized, float* h_lattice,
+                          float* h_bias_field, n4_report* report) {
+    if (!h || !p || !h_im || !h_im_normalized) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    uint32_t nc[3];
+    n4_status s = n4_final_ncpt(p, nc);
+    if (s != N4_OK) return s;
+    const size_t n = (size_t)fx * fy * fz;
+    const size_t L = (size_t)nc[0] * nc[1] * nc[2];
+    void *d_im = nullptr, *d_mask = nullptr, *d_out = nullptr, *d_lat = nullptr, *d_bias = nullptr;
+    s = N4_OK;
+    if ((s = n4_malloc(h, &d_im, n * 4)) != N4_OK) goto done;
+    if (h_mask && (s = n4_malloc(h, &d_mask, n * 4)) != N4_OK) goto done;
+    if ((s = n4_malloc(h, &d_out, n * 4)) != N4_OK) goto done;
+    if ((s = n4_malloc(h, &d_lat, L * 4)) != N4_OK) goto done;
+    if (h_bias_field && (s = n4_malloc(h, &d_bias, n * 4)) != N4_OK) goto done;
+    if ((s = n4_memcpy_htod(h, d_im, h_im, n * 4)) != N4_OK) goto done;
+    if (h_mask && (s = n4_memcpy_htod(h, d_mask, h_mask, n * 4)) != N4_OK) goto done;
+    s = n4_correct(h, (const float*)d_im, d_mask, h_mask ? N4_MASK_F32 : N4_MASK_NONE, fx, fy, fz, p,
+                   (float*)d_out, (float*)d_lat, (float*)d_bias, nullptr, report, nullptr);
+    if (s != N4_OK) goto done;
+    if ((s = n4_memcpy_dtoh(h, h_im_normalized, d_out, n * 4)) != N4_OK) goto done;
+    if (h_lattice && (s = n4_memcpy_dtoh(h, h_lattice, d_lat, L * 4)) != N4_OK) goto done;
+    if (h_bias_field && (s = n4_memcpy_dtoh(h, h_bias_field, d_bias, n * 4)) != N4_OK) goto done;
+done:
+    n4_free(h, d_im);
+    n4_free(h, d_mask);
+    n4_free(h, d_out);
+    n4_free(h, d_lat);
+    n4_free(h, d_bias);
+    return s;
+}
+
+// ----------------------------------------------------------- stage calls --
+// Flat stage plans reuse the engine's kernels on a 1-D array cut into
+// kChunkVox-voxel slabs.
+static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const float* x, const uint8_t* m,
+                           bool clear_state = true) {
+    n4_params p;
+    n4_params_default(&p);
+    p.number_of_histogram_bins = (uint32_t)nbins;
+    p.number_of_fitting_levels = 1;
+    const int nslab = (int)((n + kChunkVox - 1) / kChunkVox);
+    VolIn v{x, m, N4_MASK_U8, {nslab, 1, kChunkVox}, 1, nullptr, nullptr, nullptr, nullptr};
+    n4_status s = build_plan(h, std::vector<VolIn>{v}, &p, P, false);
+    if (s != N4_OK) return s;
+    VolDesc& D = P->desc[0];
+    D.logU = const_cast<float*>(x);
+    D.wmask = const_cast<uint8_t*>(m);
+    D.nvox_w = (int64_t)n;
+    memcpy(h->staging.data() + P->off_desc, &D, sizeof D);
+    (void)clear_state;
+    H2D(h->arena, h->staging.data(), P->upload_bytes, nullptr);
+    return N4_OK;
+}
+
+n4_status n4_histogram(n4_handle* h, const float* d_x, const uint8_t* d_mask, uint64_t n, uint32_t nbins,
+                       double* out_mms, double* d_hist, int32_t* d_bin_index) {
+    g_err.clear();
+    if (!h || !d_x || !d_mask || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
+    if (nbins < 2 || nbins > (uint32_t)kMaxBins) return fail(N4_ERR_INVALID_ARG, "nbins out of range");
+    Plan P;
+    n4_status s = flat_plan(h, n, (int)nbins, &P, d_x, d_mask);
+    if (s != N4_OK) return s;
+    VolDesc* dd = dev_desc(h, P);
+    Slab* sl = dev_wslab(h, P);
+    LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatMinMax, d_x, nullptr, nullptr, P.rp, nullptr));
+    LAUNCH(launch_hist(dd, sl, P.n_wslab, P.rp, d_bin_index, P.lds_hist, nullptr));
+    VolState st;
+    D2H(&st, dev_state(h, P), sizeof st, nullptr);
+    std::vector<uint64_t> hp((size_t)P.n_wslab * nbins);
+    D2H(hp.data(), P.desc[0].hpart, hp.size() * 8, nullptr);
+    SYNC(nullptr);
+    std::vector<double> H(nbins, 0.0);
+    if (!st.degenerate)
+        for (uint32_t b = 0; b < nbins; ++b) {
+            uint64_t acc = 0;
+            for (int xx = 0; xx < P.n_wslab; ++xx) acc += hp[(size_t)xx * nbins + b];
+            H[b] = (double)acc * 0x1p-32;
+        }
+    if (out_mms) {
+        out_mms[0] = st.bin_min;
+        out_mms[1] = st.bin_max;
+        out_mms[2] = st.slope;
+    }
+    if (d_hist) H2D(d_hist, H.data(), sizeof(double) * nbins, nullptr);
+    SYNC(nullptr);
+    return N4_OK;
+}
+
+n4_status n4_sharpen_image(n4_handle* h, const float* d_before, const uint8_t* d_mask, uint64_t n, uint32_t nbins,
+                           double wiener_noise, double fwhm, float* d_after, double* d_etable) {
+    g_err.clear();
+    if (!h || !d_before || !d_mask || !d_after || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
+    if (nbins < 2 || nbins > (uint32_t)kMaxBins) return fail(N4_ERR_INVALID_ARG, "nbins out of range");
+    if (!(fwhm > 0.0) || !(wiener_noise >= 0.0)) return fail(N4_ERR_INVALID_ARG, "bad Wiener parameters");
+    Plan P;
+    n4_status s = flat_plan(h, n, (int)nbins, &P, d_before, d_mask);
+    if (s != N4_OK) return s;
+    RunParams rp = P.rp;
+    rp.noise = wiener_noise;
+    rp.fwhm = fwhm;
+    VolDesc* dd = dev_desc(h, P);
+    Slab* sl = dev_wslab(h, P);
+    LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatMinMax, d_before, nullptr, nullptr, rp, nullptr));
+    LAUNCH(launch_hist(dd, sl, P.n_wslab, rp, nullptr, P.lds_hist, nullptr));
+    LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatRemap, d_before, nullptr, d_after, rp, nullptr));
+    if (d_etable) D2D(d_etable, P.desc[0].etab, sizeof(double) * nbins, nullptr);
+    SYNC(nullptr);
+    return N4_OK;
+}
+
+n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_mask, const uint32_t ncpt[3],
+                           const uint32_t dims[3], double eps, float* d_lattice) {
+    g_err.clear();
+    if (!h || !d_field || !d_mask || !ncpt || !dims || !d_lattice) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    n4_params p;
+    n4_params_default(&p);
+    p.number_of_fitting_levels = 1;
+    p.fit_eps = eps;
+    for (int d = 0; d < 3; ++d) p.ncpt[d] = ncpt[d];
+    n4_status s = check_params(&p);
+    if (s != N4_OK) return s;
+    VolIn v{d_field, d_mask, N4_MASK_U8, {(int)dims[0], (int)dims[1], (int)dims[2]}, 1, nullptr, nullptr, nullptr, nullptr};
+    for (int d = 0; d < 3; ++d)
+        if (v.F[d] < 1) return fail(N4_ERR_INVALID_ARG, "empty dimension");
+    Plan P;
+    s = build_plan(h, std::vector<VolIn>{v}, &p, &P, false);
+    if (s != N4_OK) return s;
+    s = check_lds(P, true, false);
+    if (s != N4_OK) return s;
+    VolDesc& D = P.desc[0];
+    D.logU = const_cast<float*>(d_field);
+    D.wmask = const_cast<uint8_t*>(d_mask);
+    memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
+    H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    VolDesc* dd = dev_desc(h, P);
+    Slab* sl = dev_wslab(h, P);
+    LAUNCH(launch_level(dd, 1, 0, P.rp, nullptr));
+    LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, kFitDen, P.rp, P.lds_fit_max[0], nullptr));
+    LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, kFitField, P.rp, P.lds_fit_max[0], nullptr));
+    D2D(d_lattice, D.lresf, sizeof(float) * D.lev[0].L, nullptr);
+    SYNC(nullptr);
+    return N4_OK;
+}
+
+n4_status n4_evaluate_bspline3d(n4_handle* h, const float* d_lattice, const uint32_t ncpt[3], const uint32_t dims[3],
+                                float* d_out) {
+    g_err.clear();
+    if (!h || !d_lattice || !ncpt || !dims || !d_out) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    n4_params p;
+    n4_params_default(&p);
+    p.number_of_fitting_levels = 1;
+    for (int d = 0; d < 3; ++d) p.ncpt[d] = ncpt[d];
+    n4_status s = check_params(&p);
+    if (s != N4_OK) return s;
+    VolIn v{nullptr, nullptr, N4_MASK_NONE, {(int)dims[0], (int)dims[1], (int)dims[2]}, 1, nullptr, nullptr, d_out, nullptr};
+    for (int d = 0; d < 3; ++d)
+        if (v.F[d] < 1) return fail(N4_ERR_INVALID_ARG, "empty dimension");
+    Plan P;
+    s = build_plan(h, std::vector<VolIn>{v}, &p, &P, true);
+    if (s != N4_OK) return s;
+    s = check_lds(P, false, true);
+    if (s != N4_OK) return s;
+    VolDesc& D = P.desc[0];
+    D.latf = const_cast<float*>(d_lattice);
+    memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
+    H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, nullptr));
+    SYNC(nullptr);
+    return N4_OK;
+}
+
+n4_status n4_convergence_measurement(n4_handle* h, const float* d_f1, const float* d_f2, const uint8_t* d_mask,
+                                     uint64_t n, double* out) {
+    g_err.clear();
+    if (!h || !d_f1 || !d_f2 || !d_mask || !out || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
+    Plan P;
+    n4_status s = flat_plan(h, n, 2, &P, d_f1, d_mask);
+    if (s != N4_OK) return s;
+    LAUNCH(launch_flat(dev_desc(h, P), dev_wslab(h, P), P.n_wslab, kFlatCV, d_f1, d_f2, nullptr, P.rp, nullptr));
+    VolState st;
+    D2H(&st, dev_state(h, P), sizeof st, nullptr);
+    SYNC(nullptr);
+    *out = st.cv;
+    return N4_OK;
+}
+
+n4_status n4_upsample_lattice3d(n4_handle* h, const float* d_lattice, const uint32_t ncpt[3], float* d_out) {
+    g_err.clear();
+    if (!h || !d_lattice || !ncpt || !d_out) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    n4_params p;
+    n4_params_default(&p);
+    p.number_of_fitting_levels = 2;
+    for (int d = 0; d < 3; ++d) p.ncpt[d] = ncpt[d];
+    n4_status s = check_params(&p);
+    if (s != N4_OK) return s;
+    // A 1x1x1 dummy grid: only the lattice buffers of the plan are used.
+    VolIn v{nullptr, nullptr, N4_MASK_NONE, {1, 1, 1}, 1, nullptr, nullptr, nullptr, nullptr};
+    Plan P;
+    s = build_plan(h, std::vector<VolIn>{v}, &p, &P, false);
+    if (s != N4_OK) return s;
+    H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    const VolDesc& D = P.desc[0];
+    const size_t L0 = (size_t)D.lev[0].L, L1 = (size_t)D.lev[1].L;
+    std::vector<float> lf(L0);
+    std::vector<double> ld(L0);
+    D2H(lf.data(), d_lattice, L0 * 4, nullptr);
+    SYNC(nullptr);
+    for (size_t i = 0; i < L0; ++i) ld[i] = lf[i];
+    H2D(D.lat[0], ld.data(), L0 * 8, nullptr);
+    LAUNCH(launch_level(dev_desc(h, P), 1, 1, P.rp, nullptr));
+    D2D(d_out, D.latf, L1 * 4, nullptr);
+    SYNC(nullptr);
+    return N4_OK;
+}
+
+static n4_status reduce_flat(n4_handle* h, uint64_t n, const float* d_in, VolState* st) {
+    if (!h || !d_in || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
+    Plan P;
+    n4_status s = flat_plan(h, n, 2, &P, d_in, nullptr);
+    if (s != N4_OK) return s;
+    LAUNCH(launch_flat(dev_desc(h, P), dev_wslab(h, P), P.n_wslab, kFlatSum, d_in, nullptr, nullptr, P.rp, nullptr));
+    D2H(st, dev_state(h, P), sizeof *st, nullptr);
+    SYNC(nullptr);
+    return N4_OK;
+}
+
+n4_status n4_reduce_sum(n4_handle* h, uint64_t n, const float* d_in, double* out) {
+    g_err.clear();
+    VolState st;
+    n4_status s = reduce_flat(h, n, d_in, &st);
+    if (s == N4_OK && out) *out = st.cv;
+    return s;
+}
+
+n4_status n4_reduce_max(n4_handle* h, uint64_t n, const float* d_in, float* out) {
+    g_err.clear();
+    VolState st;
+    n4_status s = reduce_flat(h, n, d_in, &st);
+    if (s == N4_OK && out) *out = (float)st.bin_max;
+    return s;
+}
+
+n4_status n4_reduce_min(n4_handle* h, uint64_t n, const float* d_in, float* out) {
+    g_err.clear();
+    VolState st;
+    n4_status s = reduce_flat(h, n, d_in, &st);
+    if (s == N4_OK && out) *out = (float)st.bin_min;
+    return s;
+}
+
+n4_status n4_profile_enable(n4_handle* h, uint32_t mask) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+    h->prof_mask = mask;
+    for (int k = 0; k < N4_K_COUNT; ++k) {
+        h->prof_ms[k] = 0.0;
+        h->prof_n[k] = 0;
+    }
+    return N4_OK;
+}
+
+n4_status n4_profile_read(n4_handle* h, uint32_t kernel, double* total_ms, uint64_t* launches) {
+    if (!h || kernel >= N4_K_COUNT) return fail(N4_ERR_INVALID_ARG, "bad handle or kernel id");
+    if (total_ms) *total_ms = h->prof_ms[kernel];
+    if (launches) *launches = h->prof_n[kernel];
+    return N4_OK;
+}
+
+// ------------------------------------------------------- memory plumbing --
+n4_status n4_malloc(n4_handle* h, void** d_ptr, size_t bytes) {
+    if (!h || !d_ptr) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    *d_ptr = nullptr;
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+#endif
+    if (bytes == 0) return N4_OK;
+    if (dev_alloc(d_ptr, bytes)) return fail(N4_ERR_NO_MEMORY, "allocation of %zu bytes failed", bytes);
+    return N4_OK;
+}
+
+n4_status n4_free(n4_handle* h, void* d_ptr) {
+    (void)h;
+    dev_free(d_ptr);
+    return N4_OK;
+}
+
+n4_status n4_memcpy_htod(n4_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h || (bytes && (!dst || !src))) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+#else
+    memcpy(dst, src, bytes);
+#endif
+    return N4_OK;
+}
+
+n4_status n4_memcpy_dtoh(n4_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h || (bytes && (!dst || !src))) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+#else
+    memcpy(dst, src, bytes);
+#endif
+    return N4_OK;
+}
+
+n4_status n4_memset(n4_handle* h, void* dst, int value, size_t bytes) {
+    if (!h || (bytes && !dst)) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipMemset(dst, value, bytes));
+#else
+    memset(dst, value, bytes);
+#endif
+    return N4_OK;
+}
+
+n4_status n4_synchronize(n4_handle* h) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+    N4_CHECK_HIP(hipDeviceSynchronize());
+#endif
+    return N4_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// n4_common.h -- structures shared by the gfx950 kernels and the host driver.
+//
+// Device memory layout (per volume, all in one arena owned by the handle):
+//   full resolution (caller's buffers): im f32, mask f32|u8, outputs f32
+//   working grid W = ceil(F/s) per axis, C order [x][y][z]:
+//     imlog f32, logU f32, wmask u8
+//   lattice: lat[2] f64 (ping-pong across levels), den f64, latf f32,
+//            lresf f32 (largest level size)
+//   per-slab partials (one slab = one working x index):
+//     part  f32 [W0][Cy*Cz]   fit contraction over (y, z) of the slab
+//     hpart u64 [W0][nbins]   fixed-point (2^-32) fractional histogram
+//     spart f64 [W0][4]       CV sums / counts and logU min / max
+//   etab f64 [nbins], state (VolState)
+// Axis tables (AxisEntry) are computed on the host in FP64 per level and per
+// axis and uploaded once per call.
+#pragma once
+#include <stdint.h>
+
+namespace n4 {
+
+constexpr int kBlock = 256;
+constexpr int kMaxLevels = 16;
+constexpr int kMaxBins = 512;
+constexpr int kChunkVox = 4096;   // working voxels staged in LDS per row chunk
+constexpr int kNumCounters = 4;   // per-pass "last slab" counters
+
+// One position along one axis at one level (A11/A12/A14 of the B-spline).
+struct AxisEntry {
+    float w[4];     // cubic B-spline weights B0..B3 at t (N4.py:174-181)
+    float p[4];     // w^3 / sum_a(w_a^2)  -- separable fit-numerator factor
+    float q[4];     // w^2                 -- separable fit-denominator factor
+    int32_t b;      // span index floor((i+0.5)/span)  (N4.py:210)
+    int32_t pad[3];
+};
+
+struct LevelDesc {
+    int32_t C[3];            // control points per axis
+    int32_t L;               // C0*C1*C2
+    const AxisEntry* ax[3];  // W[d] entries (working grid)
+    const int32_t* sup[3];   // 2*C[d]: [lo, hi) positions whose support holds c
+};
+
+struct VolState {
+    double bin_min, bin_max, slope;   // histogram range of logU over the mask
+    double cv;                        // last convergence measurement
+    int32_t degenerate;               // empty mask or max == min: no sharpening
+    int32_t done;                     // current level finished
+    int32_t iter;                     // iterations executed at current level
+    int32_t level;
+    uint32_t cnt[kNumCounters];       // per-pass slab-completion counters
+    uint64_t n_mask_full, n_mask_work;
+    int32_t iters[kMaxLevels];
+    double lastcv[kMaxLevels];
+};
+
+struct VolDesc {
+    // full resolution
+    const float* im;
+    const void* mask;
+    int32_t mask_kind;   // n4_mask_kind
+    int32_t s;           // shrink factor
+    float* out_norm;
+    float* out_bias;
+    float* out_logbias;
+    float* out_lattice;
+    int32_t F[3];
+    int32_t W[3];
+    int64_t nvox_w;      // W0*W1*W2 (flat stage calls may use fewer)
+    // working grid
+    float* imlog;
+    float* logU;
+    uint8_t* wmask;
+    // lattice
+    double* lat[2];
+    double* den;
+    float* latf;
+    float* lresf;
+    // partials
+    float* part;
+    uint64_t* hpart;
+    double* spart;
+    double* etab;
+    VolState* st;
+    int32_t* done_flag;        // mirror of st->done in a dense per-batch array
+    LevelDesc lev[kMaxLevels];
+    const AxisEntry* fax[3];   // full-res axis tables for the final lattice
+    int32_t nlevels;
+    int32_t max_iter[kMaxLevels];
+    int32_t part_stride;       // floats per slab in part (max Cy*Cz)
+};
+
+struct RunParams {
+    int32_t nbins;
+    int32_t P;        // padded FFT length (power of two)
+    int32_t log2P;
+    int32_t offset;   // histogram offset in the padded array
+    double noise;     // Wiener filter noise
+    double fwhm;      // bias field FWHM
+    double thr;       // convergence threshold
+    double eps;       // fit denominator eps
+    double low;       // low value threshold
+    float low_f;      // largest float <= low: x > low_f  <=>  (double)x > low
+    int32_t fixed_iters;
+};
+
+struct Slab {
+    int32_t vol;
+    int32_t x;
+};
+
+enum FitMode : int32_t { kFitNum = 0, kFitDen = 1, kFitField = 2 };
+
+}  // namespace n4

@@ -1,0 +1,210 @@
+/*
+ * n4_capi.h -- C ABI of the MI355X (gfx950) N4 bias-field-correction library
+ * (libn4amd.so).  This is the drop-in boundary: plain pointers and sizes,
+ * device pointers unless a name says host, status codes instead of exit().
+ *
+ * Layout of every volume: float32, C order [x][y][z] with z fastest, x the
+ * slowest axis (the reference's device layout, cudaN4.cu:476,562; its CLI
+ * passes ITK (size[2], size[1], size[0]) as (fx, fy, fz), main.cpp:153-155).
+ *
+ * Each entry point below names the reference interface it replaces.
+ */
+#ifndef N4_CAPI_H
+#define N4_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define N4_MAX_LEVELS 16
+#define N4_MAX_BINS 512
+
+typedef enum n4_status {
+    N4_OK = 0,
+    N4_ERR_INVALID_ARG = 1, /* bad size, null pointer, parameter out of range */
+    N4_ERR_HIP = 2,         /* a HIP runtime call failed */
+    N4_ERR_NO_MEMORY = 3,   /* device allocation failed */
+    N4_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+    N4_ERR_INTERNAL = 5
+} n4_status;
+
+typedef enum n4_mask_kind {
+    N4_MASK_F32 = 0,  /* float32, nonzero = foreground (cudaN4.h:346) */
+    N4_MASK_U8 = 1,   /* uint8, nonzero = foreground */
+    N4_MASK_NONE = 2  /* every voxel foreground (main.cpp:137-140) */
+} n4_mask_kind;
+
+typedef enum n4_precision {
+    N4_PREC_FP32 = 0,       /* FP32 direct fit / reconstruction (the drop-in) */
+    N4_PREC_BF16 = 1,       /* separable MFMA contractions, BF16 in, FP32 acc */
+    N4_PREC_SPLIT_BF16 = 2  /* hi+lo split BF16 on MFMA, FP32 accumulate */
+} n4_precision;
+
+/* Flags for n4_params.flags */
+#define N4_FLAG_FIXED_ITERATIONS 0x1u /* ignore the convergence threshold */
+
+/* Parameters.  Replaces struct N4Param (cudaN4.h:24-94); N4.py:276-279,
+ * 292, 307-311 for the per-level iteration list and fixed constants.
+ * Floating-point parameters are double so that 0.15 / 0.01 / 0.001 are the
+ * same numbers N4.py uses. */
+typedef struct n4_params {
+    uint32_t number_of_histogram_bins;          /* cudaN4.h:30  (200)   */
+    double wiener_filter_noise;                 /* cudaN4.h:38  (0.01)  */
+    double bias_field_fwhm;                     /* cudaN4.h:44  (0.15)  */
+    uint32_t ncpt[3];                           /* cudaN4.h:60-62 (4,4,4) x,y,z */
+    uint32_t number_of_fitting_levels;          /* cudaN4.h:71  (4)     */
+    uint32_t max_iterations[N4_MAX_LEVELS];     /* cudaN4.h:75  (50)    */
+    double convergence_threshold;               /* cudaN4.h:84  (0.001) */
+    double low_value;                           /* cudaN4.h:85  (10)    */
+    uint32_t shrink_factor;                     /* working grid = ceil(full/s) (1) */
+    double fit_eps;                             /* N4.py:236 denominator eps (1e-3) */
+    uint32_t precision;                         /* n4_precision (FP32)  */
+    uint32_t flags;                             /* N4_FLAG_*            */
+} n4_params;
+
+/* Per-volume run record (no counterpart in the reference, which only prints
+ * "Level %d, iter %d", cudaN4.cu:1530). */
+typedef struct n4_report {
+    uint32_t iterations[N4_MAX_LEVELS];   /* iterations executed per level */
+    double last_convergence[N4_MAX_LEVELS];
+    uint64_t n_mask_full;                 /* foreground voxels after threshold */
+    uint64_t n_mask_work;                 /* foreground voxels of the working grid */
+    uint32_t work_dims[3];
+    uint32_t final_ncpt[3];
+    float elapsed_ms;                     /* device time of the whole call */
+} n4_report;
+
+/* One volume of a batch (no counterpart in the reference). */
+typedef struct n4_volume {
+    const float *d_im;           /* input image, dims[0]*dims[1]*dims[2] */
+    const void *d_mask;          /* mask (kind below), may be NULL with N4_MASK_NONE */
+    uint32_t mask_kind;          /* n4_mask_kind */
+    uint32_t dims[3];            /* fx, fy, fz */
+    uint32_t shrink_factor;      /* 0 = use n4_params.shrink_factor */
+    float *d_im_normalized;      /* output, required */
+    float *d_lattice;            /* output final lattice, nullable */
+    float *d_bias_field;         /* output exp(log bias), nullable */
+    float *d_log_bias_field;     /* output log bias, nullable */
+} n4_volume;
+
+typedef struct n4_handle n4_handle;
+
+/* ---- library / handle ------------------------------------------------- */
+const char *n4_version(void);
+const char *n4_status_string(n4_status s);
+/* Message of the last failed call on this thread ("" if none). */
+const char *n4_last_error(void);
+/* Default parameters: N4Param's defaults (cudaN4.h:24-94), shrink 1, eps 1e-3. */
+n4_status n4_params_default(n4_params *p);
+/* Final-level control points per axis: (ncpt-3)*2^(levels-1)+3
+ * (main.cpp:83-92, cudaN4.cu:1778-1788). */
+n4_status n4_final_ncpt(const n4_params *p, uint32_t out_ncpt[3]);
+/* Handle = device + workspace + stream-ordered state.  Replaces
+ * N4Data::assign (cudaN4.h:132-174): no FFTW plans, nothing on the host. */
+n4_status n4_create(n4_handle **h, int device);
+/* Replaces N4Data::clean (cudaN4.h:177-193). */
+n4_status n4_destroy(n4_handle *h);
+
+/* ---- whole correction -------------------------------------------------- */
+/* Replaces N4() (cudaN4.h:343-365, cudaN4.cu:1418-1747).  Inputs are not
+ * modified (the reference thresholds d_im/d_mask in place, cudaN4.cu:1459).
+ * d_lattice (nullable) receives the final-level lattice, n4_final_ncpt.
+ * stream: a hipStream_t, or NULL for the null stream.  The call returns
+ * after the work completes (the report needs the iteration counts). */
+n4_status n4_correct(n4_handle *h, const float *d_im, const void *d_mask,
+                     uint32_t mask_kind, uint32_t fx, uint32_t fy, uint32_t fz,
+                     const n4_params *p, float *d_im_normalized,
+                     float *d_lattice, float *d_bias_field,
+                     float *d_log_bias_field, n4_report *report, void *stream);
+
+/* Replaces testN4 (cudaN4.h:367-387, cudaN4.cu:1751-1847): host in/out. */
+n4_status n4_correct_host(n4_handle *h, const float *h_im, const float *h_mask,
+                          uint32_t fx, uint32_t fy, uint32_t fz,
+                          const n4_params *p, float *h_im_normalized,
+                          float *h_lattice, float *h_bias_field,
+                          n4_report *report);
+
+/* Many volumes of uneven size in one launch per stage (no counterpart in
+ * the reference).  reports: nvol entries or NULL. */
+n4_status n4_correct_batch(n4_handle *h, const n4_volume *vols, uint32_t nvol,
+                           const n4_params *p, n4_report *reports, void *stream);
+
+/* ---- stage entry points (device pointers, synchronous) ----------------- */
+/* Masked min/max + fractional histogram (cudaN4.cu:703-731).
+ * d_mask: uint8.  out_minmax_slope (host, 3 doubles), d_hist (nbins doubles,
+ * device), d_bin_index (device int32 per voxel, -1 where unmasked; nullable). */
+n4_status n4_histogram(n4_handle *h, const float *d_x, const uint8_t *d_mask,
+                       uint64_t n, uint32_t nbins, double *out_minmax_slope,
+                       double *d_hist, int32_t *d_bin_index);
+
+/* Replaces sharpenImage (cudaN4.h:198-234): histogram, Wiener deconvolution
+ * (in-LDS FP64 FFT, no FFTW) and E[u|v] remap.  d_after = 0 where unmasked.
+ * d_etable (nullable, device, nbins doubles). */
+n4_status n4_sharpen_image(n4_handle *h, const float *d_before,
+                           const uint8_t *d_mask, uint64_t n, uint32_t nbins,
+                           double wiener_noise, double fwhm, float *d_after,
+                           double *d_etable);
+
+/* Replaces FitBspline3D (cudaN4.h:313-325) with N4.py semantics
+ * (lattice = num/(den+eps), N4.py:236).  d_mask: uint8. */
+n4_status n4_fit_bspline3d(n4_handle *h, const float *d_field,
+                           const uint8_t *d_mask, const uint32_t ncpt[3],
+                           const uint32_t dims[3], double eps,
+                           float *d_lattice);
+
+/* Replaces EvaluateBspline3D (cudaN4.h:288-298). */
+n4_status n4_evaluate_bspline3d(n4_handle *h, const float *d_lattice,
+                                const uint32_t ncpt[3], const uint32_t dims[3],
+                                float *d_out);
+
+/* Replaces calculateConvergenceMeasurement (cudaN4.h:249-257); ddof 0
+ * (N4.py:125).  Result written to *out (host). */
+n4_status n4_convergence_measurement(n4_handle *h, const float *d_field1,
+                                     const float *d_field2,
+                                     const uint8_t *d_mask, uint64_t n,
+                                     double *out);
+
+/* Replaces upsampleLattice3D_gpu (cudaN4.h:269-276): (n+3) -> (2n+3). */
+n4_status n4_upsample_lattice3d(n4_handle *h, const float *d_lattice,
+                                const uint32_t ncpt[3], float *d_out);
+
+/* Replace Reducer::reduce_{sum,max,min}_wrapper (reducer.h:7-9); FP64
+ * accumulation, wave64, single launch, result to host. */
+n4_status n4_reduce_sum(n4_handle *h, uint64_t n, const float *d_in, double *out);
+n4_status n4_reduce_max(n4_handle *h, uint64_t n, const float *d_in, float *out);
+n4_status n4_reduce_min(n4_handle *h, uint64_t n, const float *d_in, float *out);
+
+/* ---- kernel timing (no counterpart in the reference) ------------------- */
+/* Kernels of the correction, for n4_profile_*. */
+typedef enum n4_kernel_id {
+    N4_K_PREP = 0,    /* threshold + log + shrink (full resolution) */
+    N4_K_LEVEL = 1,   /* level setup / lattice refinement */
+    N4_K_HIST = 2,    /* fractional histogram + Wiener tail */
+    N4_K_FIT_DEN = 3, /* fit denominator (once per level) */
+    N4_K_FIT = 4,     /* remap + residual + fit numerator + lattice tail */
+    N4_K_EVAL = 5,    /* field + convergence + logU update */
+    N4_K_RECON = 6,   /* full-resolution field + normalisation */
+    N4_K_COUNT = 7
+} n4_kernel_id;
+/* Time every launch of the kernels in kernel_mask (bit = n4_kernel_id) with
+ * HIP events on the launch stream; clears the accumulators. 0 disables. */
+n4_status n4_profile_enable(n4_handle *h, uint32_t kernel_mask);
+/* Accumulated device time and launch count of one kernel since enable. */
+n4_status n4_profile_read(n4_handle *h, uint32_t kernel, double *total_ms, uint64_t *launches);
+
+/* ---- device memory plumbing (for hosts without their own HIP runtime) -- */
+n4_status n4_device_count(int *count);
+n4_status n4_malloc(n4_handle *h, void **d_ptr, size_t bytes);
+n4_status n4_free(n4_handle *h, void *d_ptr);
+n4_status n4_memcpy_htod(n4_handle *h, void *d_dst, const void *h_src, size_t bytes);
+n4_status n4_memcpy_dtoh(n4_handle *h, void *h_dst, const void *d_src, size_t bytes);
+n4_status n4_memset(n4_handle *h, void *d_dst, int value, size_t bytes);
+n4_status n4_synchronize(n4_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
