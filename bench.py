@@ -193,7 +193,7 @@ def main():
 
     # plain single-volume FP32 path over the same volumes
     if not args.no_plain:
-        lib.profile_enable(0)
+        lib.profile_enable([])
         rep1 = n4amd.Report()
 
         def plain_pass():

@@ -225,7 +225,8 @@ struct Plan {
     int n_wslab = 0, n_fslab = 0;
     std::vector<int> max_iter;  // per level (max over volumes)
     size_t lds_fit_max[kMaxLevels] = {0}, lds_eval_max[kMaxLevels] = {0};
-    size_t lds_hist = 0, lds_recon = 0;
+    int cx_max[kMaxLevels] = {0};
+    size_t lds_hist = 0, lds_wiener = 0, lds_recon = 0;
 };
 
 static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
@@ -284,7 +285,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
 
     // per-volume working buffers
     struct Offs {
-        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, hpart, spart, etab;
+        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, hist, spart, etab;
     };
     std::vector<Offs> off(nv);
     const int nb = P->rp.nbins;
@@ -301,7 +302,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.latf = lay.take(sizeof(float) * I.Lmax);
         o.lresf = lay.take(sizeof(float) * I.Lmax);
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
-        o.hpart = lay.take(sizeof(uint64_t) * I.W[0] * nb);
+        o.hist = lay.take(sizeof(uint64_t) * nb);
         o.spart = lay.take(sizeof(double) * 4 * I.W[0]);
         o.etab = lay.take(sizeof(double) * nb);
     }
@@ -345,7 +346,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.latf = (float*)(base + o.latf);
         D.lresf = (float*)(base + o.lresf);
         D.part = (float*)(base + o.part);
-        D.hpart = (uint64_t*)(base + o.hpart);
+        D.hist = (uint64_t*)(base + o.hist);
         D.spart = (double*)(base + o.spart);
         D.etab = (double*)(base + o.etab);
         D.st = (VolState*)(base + P->off_state) + v;
@@ -381,10 +382,12 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         for (int l = 0; l < L; ++l) {
             P->lds_fit_max[l] = std::max(P->lds_fit_max[l], lds_fit(nb, D, l));
             P->lds_eval_max[l] = std::max(P->lds_eval_max[l], lds_eval(D, l));
+            P->cx_max[l] = std::max(P->cx_max[l], D.lev[l].C[0]);
         }
         if (need_tables_full) P->lds_recon = std::max(P->lds_recon, lds_recon(D));
     }
-    P->lds_hist = lds_hist(nb, P->rp.P);
+    P->lds_hist = lds_hist(nb);
+    P->lds_wiener = lds_wiener(P->rp.P);
     memcpy(hb + P->off_desc, P->desc.data(), sizeof(VolDesc) * nv);
     // VolState region stays zero: counters start at 0.
     return N4_OK;
@@ -393,7 +396,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
 // LDS of every kernel a grid plan launches must fit one workgroup.
 static const size_t kLdsCap = 64 * 1024;
 static n4_status check_lds(const Plan& P, bool grid_kernels, bool recon) {
-    size_t worst = P.lds_hist;
+    size_t worst = std::max(P.lds_hist, P.lds_wiener);
     if (grid_kernels)
         for (int l = 0; l < P.nlevels; ++l) worst = std::max(worst, std::max(P.lds_fit_max[l], P.lds_eval_max[l]));
     if (recon) worst = std::max(worst, P.lds_recon);
@@ -586,21 +589,31 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in, const n4_
     Slab* wsl = dev_wslab(h, P);
     Slab* fsl = dev_fslab(h, P);
     const RunParams rp = P.rp;
-    TIMED(N4_K_PREP, launch_prep(dd, wsl, P.n_wslab, rp, stream));
+    TIMED(N4_K_PREP, launch_prep(dd, wsl, P.n_wslab, 0, rp, stream));
+    int prev_iters = 0;
     for (int l = 0; l < P.nlevels; ++l) {
-        TIMED(N4_K_LEVEL, launch_level(dd, P.nvol, l, rp, stream));
+        TIMED(N4_K_LEVEL, launch_level(dd, P.nvol, l, prev_iters, rp, stream));
         TIMED(N4_K_FIT_DEN, launch_fit(dd, wsl, P.n_wslab, l, kFitDen, rp, P.lds_fit_max[l], stream));
-        for (int it = 0; it < P.max_iter[l]; ++it) {
-            TIMED(N4_K_HIST, launch_hist(dd, wsl, P.n_wslab, rp, nullptr, P.lds_hist, stream));
+        TIMED(N4_K_LATTICE, launch_lattice(dd, P.nvol, P.cx_max[l], l, kFitDen, rp, stream));
+        int it = 0;
+        for (; it < P.max_iter[l]; ++it) {
+            TIMED(N4_K_HIST, launch_hist(dd, wsl, P.n_wslab, l, it, rp, nullptr, P.lds_hist, stream));
+            TIMED(N4_K_WIENER, launch_wiener(dd, P.nvol, rp, P.lds_wiener, stream));
             TIMED(N4_K_FIT, launch_fit(dd, wsl, P.n_wslab, l, kFitNum, rp, P.lds_fit_max[l], stream));
+            TIMED(N4_K_LATTICE, launch_lattice(dd, P.nvol, P.cx_max[l], l, kFitNum, rp, stream));
             TIMED(N4_K_EVAL, launch_eval(dd, wsl, P.n_wslab, l, rp, P.lds_eval_max[l], stream));
-            if (it + 1 == P.max_iter[l]) break;
+            if (it + 1 == P.max_iter[l]) continue;
             bool all_done = false;
             s = poll_done(h, P.nvol, l, it, stream, &all_done);
             if (s != N4_OK) return s;
-            if (all_done) break;
+            if (all_done) {
+                ++it;
+                break;
+            }
         }
+        prev_iters = it;
     }
+    TIMED(N4_K_LEVEL, launch_level(dd, P.nvol, P.nlevels, prev_iters, rp, stream));
     TIMED(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, stream));
     for (int v = 0; v < P.nvol; ++v)
         if (in[v].out_lattice) {
@@ -798,8 +811,7 @@ done:
 // ----------------------------------------------------------- stage calls --
 // Flat stage plans reuse the engine's kernels on a 1-D array cut into
 // kChunkVox-voxel slabs.
-static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const float* x, const uint8_t* m,
-                           bool clear_state = true) {
+static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const float* x, const uint8_t* m) {
     n4_params p;
     n4_params_default(&p);
     p.number_of_histogram_bins = (uint32_t)nbins;
@@ -813,8 +825,28 @@ static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const f
     D.wmask = const_cast<uint8_t*>(m);
     D.nvox_w = (int64_t)n;
     memcpy(h->staging.data() + P->off_desc, &D, sizeof D);
-    (void)clear_state;
     H2D(h->arena, h->staging.data(), P->upload_bytes, nullptr);
+    return N4_OK;
+}
+
+// Host-side reduction of the flat kernels' slab partials.
+static n4_status read_partials(n4_handle* h, const Plan& P, std::vector<double>* sp) {
+    sp->assign((size_t)4 * P.n_wslab, 0.0);
+    D2H(sp->data(), P.desc[0].spart, sizeof(double) * sp->size(), nullptr);
+    SYNC(nullptr);
+    (void)h;
+    return N4_OK;
+}
+
+// Min/max + histogram (+ Wiener E table) of a flat array: flat min/max
+// partials -> level 0 setup (mask count, cleared accumulators) -> hist ->
+// wiener.
+static n4_status flat_hist(n4_handle* h, const Plan& P, const float* d_x, RunParams rp, int32_t* d_bin_index) {
+    VolDesc* dd = dev_desc(h, P);
+    Slab* sl = dev_wslab(h, P);
+    LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatMinMax, d_x, nullptr, nullptr, rp, nullptr));
+    LAUNCH(launch_level(dd, 1, 0, 0, rp, nullptr));
+    LAUNCH(launch_hist(dd, sl, P.n_wslab, 0, 0, rp, d_bin_index, P.lds_hist, nullptr));
     return N4_OK;
 }
 
@@ -826,22 +858,16 @@ n4_status n4_histogram(n4_handle* h, const float* d_x, const uint8_t* d_mask, ui
     Plan P;
     n4_status s = flat_plan(h, n, (int)nbins, &P, d_x, d_mask);
     if (s != N4_OK) return s;
-    VolDesc* dd = dev_desc(h, P);
-    Slab* sl = dev_wslab(h, P);
-    LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatMinMax, d_x, nullptr, nullptr, P.rp, nullptr));
-    LAUNCH(launch_hist(dd, sl, P.n_wslab, P.rp, d_bin_index, P.lds_hist, nullptr));
+    s = flat_hist(h, P, d_x, P.rp, d_bin_index);
+    if (s != N4_OK) return s;
     VolState st;
     D2H(&st, dev_state(h, P), sizeof st, nullptr);
-    std::vector<uint64_t> hp((size_t)P.n_wslab * nbins);
-    D2H(hp.data(), P.desc[0].hpart, hp.size() * 8, nullptr);
+    std::vector<uint64_t> hb(nbins);
+    D2H(hb.data(), P.desc[0].hist, hb.size() * 8, nullptr);
     SYNC(nullptr);
     std::vector<double> H(nbins, 0.0);
     if (!st.degenerate)
-        for (uint32_t b = 0; b < nbins; ++b) {
-            uint64_t acc = 0;
-            for (int xx = 0; xx < P.n_wslab; ++xx) acc += hp[(size_t)xx * nbins + b];
-            H[b] = (double)acc * 0x1p-32;
-        }
+        for (uint32_t b = 0; b < nbins; ++b) H[b] = (double)hb[b] * 0x1p-32;
     if (out_mms) {
         out_mms[0] = st.bin_min;
         out_mms[1] = st.bin_max;
@@ -864,10 +890,11 @@ n4_status n4_sharpen_image(n4_handle* h, const float* d_before, const uint8_t* d
     RunParams rp = P.rp;
     rp.noise = wiener_noise;
     rp.fwhm = fwhm;
+    s = flat_hist(h, P, d_before, rp, nullptr);
+    if (s != N4_OK) return s;
     VolDesc* dd = dev_desc(h, P);
     Slab* sl = dev_wslab(h, P);
-    LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatMinMax, d_before, nullptr, nullptr, rp, nullptr));
-    LAUNCH(launch_hist(dd, sl, P.n_wslab, rp, nullptr, P.lds_hist, nullptr));
+    LAUNCH(launch_wiener(dd, 1, rp, P.lds_wiener, nullptr));
     LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatRemap, d_before, nullptr, d_after, rp, nullptr));
     if (d_etable) D2D(d_etable, P.desc[0].etab, sizeof(double) * nbins, nullptr);
     SYNC(nullptr);
@@ -900,9 +927,11 @@ n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_
     H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
     VolDesc* dd = dev_desc(h, P);
     Slab* sl = dev_wslab(h, P);
-    LAUNCH(launch_level(dd, 1, 0, P.rp, nullptr));
+    LAUNCH(launch_level(dd, 1, 0, 0, P.rp, nullptr));
     LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, kFitDen, P.rp, P.lds_fit_max[0], nullptr));
+    LAUNCH(launch_lattice(dd, 1, P.cx_max[0], 0, kFitDen, P.rp, nullptr));
     LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, kFitField, P.rp, P.lds_fit_max[0], nullptr));
+    LAUNCH(launch_lattice(dd, 1, P.cx_max[0], 0, kFitField, P.rp, nullptr));
     D2D(d_lattice, D.lresf, sizeof(float) * D.lev[0].L, nullptr);
     SYNC(nullptr);
     return N4_OK;
@@ -943,10 +972,23 @@ n4_status n4_convergence_measurement(n4_handle* h, const float* d_f1, const floa
     n4_status s = flat_plan(h, n, 2, &P, d_f1, d_mask);
     if (s != N4_OK) return s;
     LAUNCH(launch_flat(dev_desc(h, P), dev_wslab(h, P), P.n_wslab, kFlatCV, d_f1, d_f2, nullptr, P.rp, nullptr));
-    VolState st;
-    D2H(&st, dev_state(h, P), sizeof st, nullptr);
-    SYNC(nullptr);
-    *out = st.cv;
+    std::vector<double> sp;
+    s = read_partials(h, P, &sp);
+    if (s != N4_OK) return s;
+    double cnt = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int x = 0; x < P.n_wslab; ++x) {
+        cnt += sp[4 * x];
+        s1 += sp[4 * x + 1];
+        s2 += sp[4 * x + 2];
+    }
+    double cvv = 0.0;
+    if (cnt > 0.0) {
+        const double mu = s1 / cnt;
+        double var = s2 / cnt - mu * mu;
+        if (var < 0.0) var = 0.0;
+        cvv = sqrt(var) / (1.0 + mu);
+    }
+    *out = cvv;
     return N4_OK;
 }
 
@@ -973,44 +1015,54 @@ n4_status n4_upsample_lattice3d(n4_handle* h, const float* d_lattice, const uint
     SYNC(nullptr);
     for (size_t i = 0; i < L0; ++i) ld[i] = lf[i];
     H2D(D.lat[0], ld.data(), L0 * 8, nullptr);
-    LAUNCH(launch_level(dev_desc(h, P), 1, 1, P.rp, nullptr));
+    LAUNCH(launch_level(dev_desc(h, P), 1, 1, 0, P.rp, nullptr));
     D2D(d_out, D.latf, L1 * 4, nullptr);
     SYNC(nullptr);
     return N4_OK;
 }
 
-static n4_status reduce_flat(n4_handle* h, uint64_t n, const float* d_in, VolState* st) {
+static n4_status reduce_flat(n4_handle* h, uint64_t n, const float* d_in, double* sum, double* mn, double* mx) {
     if (!h || !d_in || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
     Plan P;
     n4_status s = flat_plan(h, n, 2, &P, d_in, nullptr);
     if (s != N4_OK) return s;
     LAUNCH(launch_flat(dev_desc(h, P), dev_wslab(h, P), P.n_wslab, kFlatSum, d_in, nullptr, nullptr, P.rp, nullptr));
-    D2H(st, dev_state(h, P), sizeof *st, nullptr);
-    SYNC(nullptr);
+    std::vector<double> sp;
+    s = read_partials(h, P, &sp);
+    if (s != N4_OK) return s;
+    double a = 0.0, lo = INFINITY, hi = -INFINITY;
+    for (int x = 0; x < P.n_wslab; ++x) {
+        a += sp[4 * x];
+        lo = std::min(lo, sp[4 * x + 2]);
+        hi = std::max(hi, sp[4 * x + 3]);
+    }
+    *sum = a;
+    *mn = lo;
+    *mx = hi;
     return N4_OK;
 }
 
 n4_status n4_reduce_sum(n4_handle* h, uint64_t n, const float* d_in, double* out) {
     g_err.clear();
-    VolState st;
-    n4_status s = reduce_flat(h, n, d_in, &st);
-    if (s == N4_OK && out) *out = st.cv;
+    double a, lo, hi;
+    n4_status s = reduce_flat(h, n, d_in, &a, &lo, &hi);
+    if (s == N4_OK && out) *out = a;
     return s;
 }
 
 n4_status n4_reduce_max(n4_handle* h, uint64_t n, const float* d_in, float* out) {
     g_err.clear();
-    VolState st;
-    n4_status s = reduce_flat(h, n, d_in, &st);
-    if (s == N4_OK && out) *out = (float)st.bin_max;
+    double a, lo, hi;
+    n4_status s = reduce_flat(h, n, d_in, &a, &lo, &hi);
+    if (s == N4_OK && out) *out = (float)hi;
     return s;
 }
 
 n4_status n4_reduce_min(n4_handle* h, uint64_t n, const float* d_in, float* out) {
     g_err.clear();
-    VolState st;
-    n4_status s = reduce_flat(h, n, d_in, &st);
-    if (s == N4_OK && out) *out = (float)st.bin_min;
+    double a, lo, hi;
+    n4_status s = reduce_flat(h, n, d_in, &a, &lo, &hi);
+    if (s == N4_OK && out) *out = (float)lo;
     return s;
 }
 

@@ -8,8 +8,8 @@
 //            lresf f32 (largest level size)
 //   per-slab partials (one slab = one working x index):
 //     part  f32 [W0][Cy*Cz]   fit contraction over (y, z) of the slab
-//     hpart u64 [W0][nbins]   fixed-point (2^-32) fractional histogram
 //     spart f64 [W0][4]       CV sums / counts and logU min / max
+//   hist  u64 [nbins]         fixed-point (2^-32) fractional histogram
 //   etab f64 [nbins], state (VolState)
 // Axis tables (AxisEntry) are computed on the host in FP64 per level and per
 // axis and uploaded once per call.
@@ -22,7 +22,6 @@ constexpr int kBlock = 256;
 constexpr int kMaxLevels = 16;
 constexpr int kMaxBins = 512;
 constexpr int kChunkVox = 4096;   // working voxels staged in LDS per row chunk
-constexpr int kNumCounters = 4;   // per-pass "last slab" counters
 
 // One position along one axis at one level (A11/A12/A14 of the B-spline).
 struct AxisEntry {
@@ -47,7 +46,6 @@ struct VolState {
     int32_t done;                     // current level finished
     int32_t iter;                     // iterations executed at current level
     int32_t level;
-    uint32_t cnt[kNumCounters];       // per-pass slab-completion counters
     uint64_t n_mask_full, n_mask_work;
     int32_t iters[kMaxLevels];
     double lastcv[kMaxLevels];
@@ -77,7 +75,7 @@ struct VolDesc {
     float* lresf;
     // partials
     float* part;
-    uint64_t* hpart;
+    uint64_t* hist;
     double* spart;
     double* etab;
     VolState* st;

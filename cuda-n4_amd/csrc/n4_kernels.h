@@ -1,12 +1,16 @@
 // n4_kernels.h -- launch interface of the gfx950 N4 kernels.
 //
-// Every kernel is one workgroup of kBlock (256) threads per "slab": one x
-// index of one volume's working (or full-resolution) grid.  A batch of
-// volumes of different sizes is one launch: the slab list names
-// (volume, x) pairs across all volumes.  Cross-slab reductions are
-// deterministic: each slab writes a partial, and the last slab of a volume
-// to finish (an agent-scope counter) reduces the partials in slab order and
-// runs the volume's tail work (Wiener step, lattice update, convergence).
+// Grid-wide kernels run one workgroup of kBlock (256) threads per "slab":
+// one x index of one volume's working (or full-resolution) grid.  A batch
+// of volumes of different sizes is one launch: the slab list names
+// (volume, x) pairs across all volumes.  Per-volume kernels (level setup,
+// Wiener step, lattice update) run one workgroup per volume (or per volume
+// and control-point plane).  No kernel communicates across its own
+// workgroups except through integer atomics: every cross-slab reduction is
+// written as per-slab partials and reduced in a fixed order by a later
+// kernel, so results are bitwise reproducible.
+//
+// One fitting iteration = hist -> wiener -> fit -> lattice -> eval.
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -15,31 +19,39 @@
 
 namespace n4 {
 
-// LDS bytes each kernel needs for one volume (host-side sizing).
+// LDS bytes each kernel needs (host-side sizing, same carve as the device).
 size_t lds_prep();
-size_t lds_hist(int nbins, int P);
+size_t lds_level();
+size_t lds_hist(int nbins);
+size_t lds_wiener(int P);
 size_t lds_fit(int nbins, const VolDesc& v, int level);
+size_t lds_lattice();
 size_t lds_eval(const VolDesc& v, int level);
 size_t lds_recon(const VolDesc& v);
-size_t lds_flat();
+size_t lds_flat(int nbins);
 
-// Launches; `stream` is a hipStream_t (void* keeps this header HIP-free).
-// grid = number of slabs (or volumes for the level kernel).
-int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, void* stream);
-int launch_level(const VolDesc* vols, int nvol, int level, RunParams rp, void* stream);
-int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, int32_t* dbg_idx,
-                size_t lds, void* stream);
-int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int mode, RunParams rp,
-               size_t lds, void* stream);
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp,
-                size_t lds, void* stream);
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp,
-                 size_t lds, void* stream);
+// `stream` is a hipStream_t (void* keeps this header HIP-free).  Return 0
+// on success, else a hipError_t value.
+int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink_hint, RunParams rp, void* stream);
+// level in [0, nlevels]: commits the previous level's last iteration
+// (prev_iters enqueued) and, if level < nlevels, sets the level up.
+int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunParams rp, void* stream);
+int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
+                int32_t* dbg_idx, size_t lds, void* stream);
+int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void* stream);
+int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int mode, RunParams rp, size_t lds,
+               void* stream);
+int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mode, RunParams rp, void* stream);
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
+                void* stream);
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
+                 void* stream);
 
 // Flat (1-D) stage helpers used by the stage entry points.  A flat array of
-// n voxels is cut into slabs of kChunkVox voxels.
+// n voxels is cut into slabs of kChunkVox voxels; each slab writes
+// spart[x] = 4 doubles (reduced on the host by the stage call).
 enum FlatOp : int32_t { kFlatMinMax = 0, kFlatSum = 1, kFlatCV = 2, kFlatRemap = 3 };
-int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const float* a,
-                const float* b, float* out, RunParams rp, void* stream);
+int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const float* a, const float* b,
+                float* out, RunParams rp, void* stream);
 
 }  // namespace n4

@@ -8,13 +8,15 @@
 //
 // Reference stages (cudaN4.cu / N4.py) and where they live here:
 //   lowthreshold + log_kernel (+ build-defined shrink) -> body_prep
-//   set_background + reduce_min/max                   -> fused into prep / eval tails
+//   set_background + reduce_min/max + CV reductions    -> per-slab partials,
+//                                                         reduced in body_hist
 //   histogramGPU_kernel (cudaN4.cu:153-220)            -> body_hist
-//   host FFTW Wiener block (cudaN4.cu:733-820)         -> wiener_tail (LDS FP64 FFT)
-//   histogramMapping + subtract + accumulate_WC2phic_and_WC2 + divide + sum_inplace
-//                                                      -> body_fit (+ fit_tail)
-//   evaluate_bspline_kernel_3D + subtract_and_exp + CV + subtract
-//                                                      -> body_eval (+ eval_tail)
+//   host FFTW Wiener block (cudaN4.cu:733-820)         -> body_wiener (LDS FP64 FFT)
+//   histogramMapping + subtract + accumulate_WC2phic_and_WC2
+//                                                      -> body_fit
+//   divide_kernel + sum_inplace_kernel                 -> body_lattice
+//   evaluate_bspline_kernel_3D + subtract_and_exp + subtract
+//                                                      -> body_eval
 //   upsample_lattice_kernel_3D (cudaN4.cu:257-339)     -> body_level
 //   evaluate at full res + exp_and_divide + exp        -> body_recon
 #include "n4_kernels.h"
@@ -40,10 +42,12 @@
 #define BLOCK_FOR(t) { const int t = (int)threadIdx.x; (void)t;
 #define BLOCK_END }
 #define BLOCK_SYNC() __syncthreads()
+#define UNROLL _Pragma("unroll")
 #else
 #define BLOCK_FOR(t) for (int t = 0; t < kBlock; ++t) {
 #define BLOCK_END }
 #define BLOCK_SYNC() do { } while (0)
+#define UNROLL
 #endif
 
 #define HD __host__ __device__ inline
@@ -66,31 +70,11 @@ HD void lds_add_u64(unsigned long long* p, unsigned long long v) {
 #endif
 }
 
-HD void fence_release() {
+HD void global_add_u64(uint64_t* p, unsigned long long v) {
 #ifdef N4_DEVICE_PASS
-    __threadfence();
-#endif
-}
-
-HD void fence_acquire() {
-#ifdef N4_DEVICE_PASS
-    __threadfence();
-#endif
-}
-
-HD uint32_t counter_arrive(uint32_t* c) {
-#ifdef N4_DEVICE_PASS
-    return atomicAdd(c, 1u);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), v);
 #else
-    return (*c)++;
-#endif
-}
-
-HD void counter_reset(uint32_t* c) {
-#ifdef N4_DEVICE_PASS
-    atomicExch(c, 0u);
-#else
-    *c = 0;
+    *p += v;
 #endif
 }
 
@@ -119,12 +103,6 @@ struct Carve {
     }
 };
 
-HD bool mask_at(const VolDesc& V, size_t i) {
-    if (V.mask_kind == 0) return reinterpret_cast<const float*>(V.mask)[i] != 0.0f;
-    if (V.mask_kind == 1) return reinterpret_cast<const uint8_t*>(V.mask)[i] != 0;
-    return true;
-}
-
 // Level-tagged done flag in pinned host memory (level+1 when the volume has
 // finished the level, 0 otherwise), read by the host's early-exit poll.
 HD void publish_done(const VolDesc& V, int level, int done) {
@@ -136,21 +114,14 @@ HD void publish_done(const VolDesc& V, int level, int done) {
 #endif
 }
 
-// Histogram range for the next sharpening (N4.py:14-15).
-HD void set_range(VolState* st, double mn, double mx, uint64_t n, int nbins) {
-    st->bin_min = mn;
-    st->bin_max = mx;
-    double slope = (mx - mn) / (double)(nbins - 1);
-    st->slope = slope;
-    st->degenerate = (n == 0 || !(slope > 0.0)) ? 1 : 0;
-}
-
 HD double red_apply(int op, double a, double b) {
     return op == 0 ? a + b : (op == 1 ? fmin(a, b) : fmax(a, b));
 }
 
 // Tree reduction of four kBlock-long LDS arrays (op: 0 sum, 1 min, 2 max);
-// results in red[0], red[kBlock], red[2*kBlock], red[3*kBlock].
+// results in red[0], red[kBlock], red[2*kBlock], red[3*kBlock].  The order
+// of operations is fixed, so every block reducing the same data gets the
+// same bits.
 HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
     BLOCK_SYNC();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
@@ -166,68 +137,114 @@ HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
     }
 }
 
-// Counts this slab as finished; true in the last slab of the volume to
-// finish (all partials of the volume are then visible to it).
-HD bool slab_arrive(uint32_t* counter, uint32_t total, int* flag) {
+// Reduces the volume's per-slab partials spart[x] = (s0, s1, min, max).
+HD void reduce_slab_partials(const VolDesc& V, int nslab, double* red) {
     BLOCK_FOR(t)
-    fence_release();
-    BLOCK_END
-    BLOCK_SYNC();
-    BLOCK_FOR(t)
-    if (t == 0) {
-        uint32_t prev = counter_arrive(counter);
-        int last = (prev + 1 == total) ? 1 : 0;
-        if (last) counter_reset(counter);
-        *flag = last;
+    double r0 = 0.0, r1 = 0.0, r2 = INFINITY, r3 = -INFINITY;
+    for (int xx = t; xx < nslab; xx += kBlock) {
+        const double* sp = V.spart + (size_t)xx * 4;
+        r0 += sp[0];
+        r1 += sp[1];
+        r2 = fmin(r2, sp[2]);
+        r3 = fmax(r3, sp[3]);
     }
+    red[t] = r0;
+    red[kBlock + t] = r1;
+    red[2 * kBlock + t] = r2;
+    red[3 * kBlock + t] = r3;
     BLOCK_END
-    BLOCK_SYNC();
-    bool last = (*flag != 0);
-    if (last) {
-        BLOCK_FOR(t)
-        fence_acquire();
-        BLOCK_END
-    }
-    return last;
+    tree_reduce4(red, 0, 0, 1, 2);
+}
+
+// Coefficient of variation of exp(old - new) over n voxels from
+// s1 = sum(u), s2 = sum(u^2), u = exp(old - new) - 1 (N4.py:121-126, ddof 0).
+HD double cv_from_sums(double s1, double s2, double n) {
+    if (!(n > 0.0)) return 0.0;
+    const double mu = s1 / n;
+    double var = s2 / n - mu * mu;
+    if (var < 0.0) var = 0.0;
+    return sqrt(var) / (1.0 + mu);
+}
+
+HD int rows_per_chunk(int Y, int Z) {
+    int r = kChunkVox / (Z > 0 ? Z : 1);
+    if (r < 1) r = 1;
+    return r < Y ? r : Y;
 }
 
 // ------------------------------------------------------------------ prep --
 // Threshold + log + shrink (N4.py:280-287; cudaN4.cu:131-147, 639-652) for
 // one working slab: each thread owns working voxels (y, z) and reads their
-// s*s*s full-resolution blocks.  Writes imlog, logU (= imlog), wmask; the
-// last slab reduces counts and the initial logU min/max.
-HD void body_prep(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
-    const Slab sl = slabs[bid];
-    const VolDesc& V = vols[sl.vol];
+// s*s*s full-resolution blocks (lanes on consecutive z read consecutive
+// s-float runs).  Writes imlog, logU (= imlog), wmask and the slab partial
+// (masked full voxels, foreground working voxels, min, max of imlog).
+// S > 0: shrink factor known at compile time (unrolled); S == 0: any.
+template <int S, int MK>
+HD bool prep_mask(const VolDesc& V, size_t fv) {
+    if (MK == 0) return reinterpret_cast<const float*>(V.mask)[fv] != 0.0f;
+    if (MK == 1) return reinterpret_cast<const uint8_t*>(V.mask)[fv] != 0;
+    return true;
+}
+
+template <int S, int MK>
+HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, int& nm, int& nb, float& acc) {
+    const int F0 = V.F[0], F1 = V.F[1], F2 = V.F[2];
+    const int ss = S > 0 ? S : V.s;
+    const int i0 = x * ss, j0 = J * ss, k0 = K * ss;
+    const bool interior = (i0 + ss <= F0) && (j0 + ss <= F1) && (k0 + ss <= F2);
+    if (S > 0 && interior) {
+        constexpr int SS = S > 0 ? S : 1;
+        UNROLL for (int a = 0; a < SS; ++a) {
+            float xv[SS * SS];
+            bool mv[SS * SS];
+            UNROLL for (int b = 0; b < SS; ++b) {
+                UNROLL for (int c = 0; c < SS; ++c) {
+                    const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
+                    xv[b * SS + c] = V.im[fv];
+                    mv[b * SS + c] = prep_mask<S, MK>(V, fv);
+                }
+            }
+            UNROLL for (int q = 0; q < SS * SS; ++q) {
+                ++nb;
+                if (mv[q] && xv[q] > rp.low_f && isfinite(xv[q])) {
+                    ++nm;
+                    acc += logf(xv[q]);
+                }
+            }
+        }
+        return;
+    }
+    const int i1 = (i0 + ss < F0) ? i0 + ss : F0;
+    const int j1 = (j0 + ss < F1) ? j0 + ss : F1;
+    const int k1 = (k0 + ss < F2) ? k0 + ss : F2;
+    for (int i = i0; i < i1; ++i)
+        for (int j = j0; j < j1; ++j) {
+            const size_t row = ((size_t)i * F1 + j) * F2;
+            for (int k = k0; k < k1; ++k) {
+                ++nb;
+                const float xv = V.im[row + k];
+                if (prep_mask<S, MK>(V, row + k) && xv > rp.low_f && isfinite(xv)) {
+                    ++nm;
+                    acc += logf(xv);
+                }
+            }
+        }
+}
+
+template <int S, int MK>
+HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsigned char* smem) {
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
-    int* flag = cv.take<int>(1);
-    const int s = V.s;
-    const int F0 = V.F[0], F1 = V.F[1], F2 = V.F[2];
     const int W1 = V.W[1], W2 = V.W[2];
     const int x = sl.x;
     BLOCK_FOR(t)
     double nfull = 0.0, nwork = 0.0;
     double mn = INFINITY, mx = -INFINITY;
-    const int i0 = x * s, i1 = (i0 + s < F0) ? i0 + s : F0;
     for (int v = t; v < W1 * W2; v += kBlock) {
         const int J = v / W2, K = v - (v / W2) * W2;
-        const int j0 = J * s, j1 = (j0 + s < F1) ? j0 + s : F1;
-        const int k0 = K * s, k1 = (k0 + s < F2) ? k0 + s : F2;
         int nm = 0, nb = 0;
         float acc = 0.0f;
-        for (int i = i0; i < i1; ++i)
-            for (int j = j0; j < j1; ++j) {
-                const size_t row = ((size_t)i * F1 + j) * F2;
-                for (int k = k0; k < k1; ++k) {
-                    ++nb;
-                    const float xv = V.im[row + k];
-                    if (mask_at(V, row + k) && xv > rp.low_f && isfinite(xv)) {
-                        ++nm;
-                        acc += logf(xv);
-                    }
-                }
-            }
+        prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
         const size_t w = ((size_t)x * W1 + J) * W2 + K;
         const bool fg = nm > 0 && 2 * nm >= nb;
         const float lg = nm > 0 ? acc / (float)nm : 0.0f;
@@ -256,35 +273,58 @@ HD void body_prep(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
         sp[3] = red[3 * kBlock];
     }
     BLOCK_END
-    if (!slab_arrive(&V.st->cnt[0], (uint32_t)V.W[0], flag)) return;
-    BLOCK_FOR(t)
-    if (t == 0) {
-        double nf = 0.0, nw = 0.0, mn = INFINITY, mx = -INFINITY;
-        for (int xx = 0; xx < V.W[0]; ++xx) {
-            const double* sp = V.spart + (size_t)xx * 4;
-            nf += sp[0];
-            nw += sp[1];
-            mn = fmin(mn, sp[2]);
-            mx = fmax(mx, sp[3]);
-        }
-        VolState* st = V.st;
-        st->n_mask_full = (uint64_t)nf;
-        st->n_mask_work = (uint64_t)nw;
-        set_range(st, mn, mx, (uint64_t)nw, rp.nbins);
-        st->level = 0;
-        st->iter = 0;
-        st->done = 0;
-        st->cv = 1e10;
+}
+
+template <int S>
+HD void body_prep_s(const VolDesc& V, const Slab sl, const RunParams& rp, unsigned char* smem) {
+    if (V.mask_kind == 0)
+        body_prep_t<S, 0>(V, sl, rp, smem);
+    else if (V.mask_kind == 1)
+        body_prep_t<S, 1>(V, sl, rp, smem);
+    else
+        body_prep_t<S, 2>(V, sl, rp, smem);
+}
+
+HD void body_prep(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    switch (V.s) {
+        case 1: body_prep_s<1>(V, sl, rp, smem); break;
+        case 2: body_prep_s<2>(V, sl, rp, smem); break;
+        case 4: body_prep_s<4>(V, sl, rp, smem); break;
+        default: body_prep_s<0>(V, sl, rp, smem); break;
     }
-    BLOCK_END
 }
 
 // ------------------------------------------------------------ level setup --
-// One block per volume.  Level 0: zero lattice.  Level l > 0: dyadic
-// refinement of the previous level's FP64 lattice (N4.py:132-165,
-// cudaN4.cu:257-339), per-axis clip bounds.
-HD void body_level(const VolDesc* vols, int level, int bid) {
+// One block per volume.
+//  1. level > 0: commits the previous level's last iteration (prev_iters
+//     enqueued by the host) unless the volume already stopped.
+//  2. level == 0: reduces the prep partials to the mask counts, zeroes the
+//     lattice and the histogram accumulator.
+//     0 < level < nlevels: dyadic refinement of the previous level's FP64
+//     lattice (N4.py:132-165, cudaN4.cu:257-339), per-axis clip bounds.
+//  level == nlevels only commits.
+HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp, int bid, unsigned char* smem) {
     const VolDesc& V = vols[bid];
+    VolState* st = V.st;
+    Carve cv{smem, 0};
+    double* red = cv.take<double>(4 * kBlock);
+    reduce_slab_partials(V, V.W[0], red);
+    BLOCK_FOR(t)
+    if (t == 0) {
+        if (level == 0) {
+            st->n_mask_full = (uint64_t)red[0];
+            st->n_mask_work = (uint64_t)red[kBlock];
+        } else if (!st->done && prev_iters > 0) {
+            const double cvv = cv_from_sums(red[0], red[kBlock], (double)st->n_mask_work);
+            st->iters[level - 1] = prev_iters;
+            st->lastcv[level - 1] = cvv;
+            st->cv = cvv;
+        }
+    }
+    BLOCK_END
+    if (level >= V.nlevels) return;
     const LevelDesc& LD = V.lev[level];
     double* dst = V.lat[level & 1];
     if (level == 0) {
@@ -294,6 +334,7 @@ HD void body_level(const VolDesc* vols, int level, int bid) {
             V.latf[l] = 0.0f;
             V.lresf[l] = 0.0f;
         }
+        for (int i = t; i < rp.nbins; i += kBlock) V.hist[i] = 0;
         BLOCK_END
     } else {
         const LevelDesc& PD = V.lev[level - 1];
@@ -338,7 +379,6 @@ HD void body_level(const VolDesc* vols, int level, int bid) {
     }
     BLOCK_FOR(t)
     if (t == 0) {
-        VolState* st = V.st;
         st->level = level;
         st->iter = 0;
         st->cv = 1e10;
@@ -350,7 +390,105 @@ HD void body_level(const VolDesc* vols, int level, int bid) {
     BLOCK_END
 }
 
-// ----------------------------------------------------------- Wiener tail --
+// ------------------------------------------------------------- histogram --
+// Prologue (every block of the volume computes identical bits): reduce the
+// eval partials of the previous iteration -> logU min/max (N4.py:14) and,
+// for it > 0, the convergence measurement of iteration it-1 and the stop
+// decision (N4.py:301-302).  The block of slab 0 commits them to the volume
+// state.  Then the fractional (linear-split) histogram of logU over the
+// mask, N4.py:19-29 (cudaN4.cu:153-220): bin index in FP64 exactly as N4.py
+// computes it; contributions accumulate as 2^-32 fixed point in LDS-private
+// u64 bins, then in the volume's global u64 bins (integer adds:
+// order-independent, so bitwise deterministic).
+HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, RunParams rp, int32_t* dbg_idx,
+                  int bid, unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    VolState* st = V.st;
+    const int nb = rp.nbins;
+    Carve cv{smem, 0};
+    double* red = cv.take<double>(4 * kBlock);
+    unsigned long long* h = cv.take<unsigned long long>(nb);
+    reduce_slab_partials(V, V.W[0], red);
+    const double n = (double)st->n_mask_work;
+    const double mn = red[2 * kBlock], mx = red[3 * kBlock];
+    double cvv = 0.0;
+    bool stop = false;
+    if (it > 0) {
+        cvv = cv_from_sums(red[0], red[kBlock], n);
+        stop = !rp.fixed_iters && !(cvv > rp.thr);
+    }
+    const double slope = (mx - mn) / (double)(nb - 1);
+    const bool degenerate = !(n > 0.0) || !(slope > 0.0);
+    if (sl.x == 0) {
+        BLOCK_FOR(t)
+        if (t == 0 && !st->done) {
+            if (it > 0) {
+                st->iters[level] = it;
+                st->lastcv[level] = cvv;
+                st->cv = cvv;
+                if (stop) {
+                    st->done = 1;
+                    publish_done(V, level, 1);
+                }
+            }
+            st->iter = it;
+            st->bin_min = mn;
+            st->bin_max = mx;
+            st->slope = slope;
+            st->degenerate = degenerate ? 1 : 0;
+        }
+        BLOCK_END
+    }
+    if (stop || degenerate) return;
+    const int YZ = V.W[1] * V.W[2];
+    const size_t base = (size_t)sl.x * YZ;
+    BLOCK_FOR(t)
+    for (int i = t; i < nb; i += kBlock) h[i] = 0ull;
+    BLOCK_END
+    BLOCK_SYNC();
+    BLOCK_FOR(t)
+    for (int v0 = 0; v0 < YZ; v0 += 4 * kBlock) {
+        float xs[4];
+        uint8_t ms[4];
+        UNROLL for (int u = 0; u < 4; ++u) {
+            const int v = v0 + u * kBlock + t;
+            const size_t w = base + v;
+            const bool ok = v < YZ && (int64_t)w < V.nvox_w;
+            ms[u] = ok ? V.wmask[w] : 0;
+            xs[u] = ok ? V.logU[w] : 0.0f;
+        }
+        UNROLL for (int u = 0; u < 4; ++u) {
+            const int v = v0 + u * kBlock + t;
+            const size_t w = base + v;
+            if (!ms[u]) {
+                if (dbg_idx && v < YZ && (int64_t)w < V.nvox_w) dbg_idx[w] = -1;
+                continue;
+            }
+            const double c = ((double)xs[u] - mn) / slope;
+            const double fl = floor(c);
+            const int idx = (fl >= 0.0 && fl < (double)nb) ? (int)fl : -1;
+            if (dbg_idx) dbg_idx[w] = idx;
+            if (idx < 0) continue;
+            const double off = c - fl;
+            if (off == 0.0) {
+                lds_add_u64(&h[idx], 1ull << 32);
+            } else if (idx < nb - 1) {
+                const unsigned long long q = (unsigned long long)rint(off * 4294967296.0);
+                lds_add_u64(&h[idx], (1ull << 32) - q);
+                lds_add_u64(&h[idx + 1], q);
+            }
+        }
+    }
+    BLOCK_END
+    BLOCK_SYNC();
+    BLOCK_FOR(t)
+    for (int i = t; i < nb; i += kBlock)
+        if (h[i]) global_add_u64(&V.hist[i], h[i]);
+    BLOCK_END
+}
+
+// ---------------------------------------------------------------- Wiener --
 // LDS FP64 Stockham radix-2 FFT over P points; returns the buffer holding
 // the result (the other one is free).
 HD D2* lds_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
@@ -377,31 +515,29 @@ HD D2* lds_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
     return a;
 }
 
-// Wiener deconvolution of the fractional histogram and the E[u|v] table,
-// N4.py:31-101 (cudaN4.cu:733-820 did this on the host with FFTW).  Two real
-// transforms share one complex FFT (V in the real part, F in the imaginary).
-HD void wiener_tail(const VolDesc& V, const RunParams& rp, unsigned char* smem) {
+// One block per volume: Wiener deconvolution of the fractional histogram
+// and the E[u|v] table, N4.py:31-101 (cudaN4.cu:733-820 did this on the
+// host with FFTW).  Two real transforms share one complex FFT (V in the real
+// part, F in the imaginary).  Reads and clears the volume's histogram.
+HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* smem) {
+    const VolDesc& V = vols[bid];
+    const VolState* st = V.st;
+    if (st->done || st->degenerate) return;
     const int P = rp.P, half = P / 2, off = rp.offset, nb = rp.nbins;
     Carve cv{smem, 0};
     D2* z = cv.take<D2>(P);
     D2* y = cv.take<D2>(P);
     D2* ff = cv.take<D2>(P);
     D2* tw = cv.take<D2>(half);
-    const double bmin = V.st->bin_min, slope = V.st->slope;
+    const double bmin = st->bin_min, slope = st->slope;
     const double scaledFWHM = rp.fwhm / slope;
     const double expF = 4.0 * log(2.0) / (scaledFWHM * scaledFWHM);
     const double scale = 2.0 * sqrt(log(2.0) / M_PI) / scaledFWHM;
-    const int nslab = V.W[0];
     BLOCK_FOR(t)
     for (int j = t; j < P; j += kBlock) {
         const int jj = (j <= half) ? j : P - j;
         const double Fj = (j == 0) ? scale : scale * exp((double)(-(int64_t)jj * jj) * expF);
-        double Vj = 0.0;
-        if (j >= off && j < off + nb) {
-            unsigned long long acc = 0;
-            for (int xx = 0; xx < nslab; ++xx) acc += V.hpart[(size_t)xx * nb + (j - off)];
-            Vj = (double)acc * 0x1p-32;
-        }
+        const double Vj = (j >= off && j < off + nb) ? (double)V.hist[j - off] * 0x1p-32 : 0.0;
         z[j] = D2{Vj, Fj};
     }
     for (int k = t; k < half; k += kBlock) {
@@ -411,6 +547,9 @@ HD void wiener_tail(const VolDesc& V, const RunParams& rp, unsigned char* smem) 
     }
     BLOCK_END
     BLOCK_SYNC();
+    BLOCK_FOR(t)
+    for (int i = t; i < nb; i += kBlock) V.hist[i] = 0;
+    BLOCK_END
     D2* Z = lds_fft(z, y, tw, P, false);
     D2* U = (Z == z) ? y : z;
     // Split Vf and Ff, Gf = conj(Ff)/(conj(Ff)Ff + noise), Uf = Vf*Re(Gf)
@@ -453,71 +592,15 @@ HD void wiener_tail(const VolDesc& V, const RunParams& rp, unsigned char* smem) 
     BLOCK_END
 }
 
-// ------------------------------------------------------------- histogram --
-// Fractional (linear-split) histogram of logU over the mask, N4.py:19-29
-// (cudaN4.cu:153-220).  Bin index in FP64 exactly as N4.py computes it;
-// contributions accumulate as 2^-32 fixed point in LDS-private u64 bins
-// (integer adds: order-independent, so bitwise deterministic).  The last
-// slab of a volume runs the Wiener tail.
-HD void body_hist(const VolDesc* vols, const Slab* slabs, RunParams rp, int32_t* dbg_idx, int bid,
-                  unsigned char* smem) {
-    const Slab sl = slabs[bid];
-    const VolDesc& V = vols[sl.vol];
-    if (V.st->done || V.st->degenerate) return;
-    const int nb = rp.nbins;
-    Carve cv{smem, 0};
-    unsigned long long* h = cv.take<unsigned long long>(nb);
-    int* flag = cv.take<int>(1);
-    const double bmin = V.st->bin_min, slope = V.st->slope;
-    const int YZ = V.W[1] * V.W[2];
-    const size_t base = (size_t)sl.x * YZ;
-    BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock) h[i] = 0ull;
-    BLOCK_END
-    BLOCK_SYNC();
-    BLOCK_FOR(t)
-    for (int v = t; v < YZ; v += kBlock) {
-        const size_t w = base + v;
-        if ((int64_t)w >= V.nvox_w) break;
-        if (!V.wmask[w]) {
-            if (dbg_idx) dbg_idx[w] = -1;
-            continue;
-        }
-        const double c = ((double)V.logU[w] - bmin) / slope;
-        const double fl = floor(c);
-        const int idx = (fl >= 0.0 && fl < (double)nb) ? (int)fl : -1;
-        if (dbg_idx) dbg_idx[w] = idx;
-        if (idx < 0) continue;
-        const double off = c - fl;
-        if (off == 0.0) {
-            lds_add_u64(&h[idx], 1ull << 32);
-        } else if (idx < nb - 1) {
-            const unsigned long long q = (unsigned long long)rint(off * 4294967296.0);
-            lds_add_u64(&h[idx], (1ull << 32) - q);
-            lds_add_u64(&h[idx + 1], q);
-        }
-    }
-    BLOCK_END
-    BLOCK_SYNC();
-    BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock) V.hpart[(size_t)sl.x * nb + i] = h[i];
-    BLOCK_END
-    if (!slab_arrive(&V.st->cnt[1], (uint32_t)V.W[0], flag)) return;
-    BLOCK_SYNC();
-    wiener_tail(V, rp, smem);
-}
-
 // ------------------------------------------------------------------- fit --
-// Remap (N4.py:103-115) + residual (N4.py:313) + separable B-spline fit
-// (N4.py:186-237 rewritten as mode-n contractions, exact because the
-// normaliser sum(w^2) factorises per axis):
+// Remap (N4.py:103-115) + residual (N4.py:313) + the (y, z) part of the
+// separable B-spline fit (N4.py:186-237 as mode-n contractions, exact
+// because the normaliser sum(w^2) factorises per axis):
 //   T1[y,cz] = sum_z g[y,z] Pz[z,cz]      (LDS, per row chunk)
 //   T2[cy,cz] += sum_y Py[y,cy] T1[y,cz]  (LDS, accumulated over the slab)
 //   part[x] = T2                          (global, per slab)
-// tail: num[cx,cy,cz] = sum_x Px[x,cx] part[x][cy,cz] in FP64, then
-//   NUM  : lres = num/(den+eps); lat += lres  (N4.py:236, 317)
-//   DEN  : den = num (g = mask, Q = w^2 weights; once per level)
-//   FIELD: lres = num/(den+eps) of the logU buffer itself (stage API)
+// Modes: NUM (g = residual, P = w^3/S), DEN (g = mask, Q = w^2; once per
+// level), FIELD (g = the logU buffer itself; stage API).
 struct FitLds {
     double* E;
     float* zw;
@@ -529,7 +612,6 @@ struct FitLds {
     float* T2;
     float* yw;
     int* yb;
-    int* flag;
 };
 
 HD size_t carve_fit(unsigned char* base, int nbins, int Z, int R, int Cy, int Cz, FitLds* o) {
@@ -545,22 +627,15 @@ HD size_t carve_fit(unsigned char* base, int nbins, int Z, int R, int Cy, int Cz
     L.T2 = c.take<float>((size_t)Cy * Cz);
     L.yw = c.take<float>((size_t)4 * R);
     L.yb = c.take<int>(R);
-    L.flag = c.take<int>(1);
     if (o) *o = L;
     return align16(c.off);
-}
-
-HD int rows_per_chunk(int Y, int Z) {
-    int r = kChunkVox / (Z > 0 ? Z : 1);
-    if (r < 1) r = 1;
-    return r < Y ? r : Y;
 }
 
 HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, RunParams rp, int bid,
                  unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    VolState* st = V.st;
+    const VolState* st = V.st;
     if (mode == kFitNum && st->done) return;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
@@ -590,33 +665,44 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
     BLOCK_SYNC();
     for (int y0 = 0; y0 < Y; y0 += R) {
         const int ny = (Y - y0 < R) ? Y - y0 : R;
+        const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int r = t; r < ny; r += kBlock) {
             const AxisEntry& e = axy[y0 + r];
             for (int a = 0; a < 4; ++a) S.yw[4 * r + a] = (mode == kFitDen) ? e.q[a] : e.p[a];
             S.yb[r] = e.b;
         }
-        for (int v = t; v < ny * Z; v += kBlock) {
-            const size_t w = ((size_t)x * Y + y0) * Z + v;
-            float gv = 0.0f;
-            if (V.wmask[w]) {
-                if (mode == kFitDen) {
-                    gv = 1.0f;
-                } else if (mode == kFitField) {
-                    gv = V.logU[w];
-                } else if (!degenerate) {
-                    const float xv = V.logU[w];
-                    double c = ((double)xv - bmin) / slope;
-                    c = fmin(fmax(c, 0.0), (double)nb);  // cudaN4.cu:242 clamp
-                    const double fl = floor(c);
-                    const int idx = (int)fl;
-                    const double sh = (idx < nb - 1)
-                                          ? S.E[idx] + (S.E[idx + 1] - S.E[idx]) * (c - fl)
-                                          : S.E[nb - 1];
-                    gv = (float)((double)xv - sh);
-                }
+        for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
+            float xs[4];
+            uint8_t ms[4];
+            UNROLL for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * kBlock + t;
+                const bool ok = v < ny * Z;
+                ms[u] = ok ? V.wmask[base + v] : 0;
+                xs[u] = (ok && mode != kFitDen) ? V.logU[base + v] : 0.0f;
             }
-            S.g[v] = gv;
+            UNROLL for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * kBlock + t;
+                if (v >= ny * Z) continue;
+                float gv = 0.0f;
+                if (ms[u]) {
+                    if (mode == kFitDen) {
+                        gv = 1.0f;
+                    } else if (mode == kFitField) {
+                        gv = xs[u];
+                    } else if (!degenerate) {
+                        double c = ((double)xs[u] - bmin) / slope;
+                        c = fmin(fmax(c, 0.0), (double)nb);  // cudaN4.cu:242 clamp
+                        const double fl = floor(c);
+                        const int idx = (int)fl;
+                        const double sh = (idx < nb - 1)
+                                              ? S.E[idx] + (S.E[idx + 1] - S.E[idx]) * (c - fl)
+                                              : S.E[nb - 1];
+                        gv = (float)((double)xs[u] - sh);
+                    }
+                }
+                S.g[v] = gv;
+            }
         }
         BLOCK_END
         BLOCK_SYNC();
@@ -650,34 +736,66 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
     BLOCK_FOR(t)
     for (int pr = t; pr < Cy * Cz; pr += kBlock) V.part[(size_t)x * V.part_stride + pr] = S.T2[pr];
     BLOCK_END
-    if (!slab_arrive(&st->cnt[2], (uint32_t)V.W[0], S.flag)) return;
-    // tail: mode-x contraction in FP64 and the lattice update
+}
+
+// --------------------------------------------------------------- lattice --
+// Mode-x contraction of the slab partials and the lattice update; one block
+// per (control-point plane cx, volume):
+//   num[cx,p] = sum_x Px[x,cx] part[x][p]   (FP64, fixed summation order)
+//   NUM  : lres = num/(den+eps); lat += lres  (N4.py:236, 317)
+//   DEN  : den = num
+//   FIELD: lres = num/(den+eps)
+// Each column p is summed by G = kBlock/CyCz threads over interleaved slabs,
+// then combined in a fixed order.
+HD void body_lattice(const VolDesc* vols, int level, int mode, RunParams rp, int cx, int vol,
+                     unsigned char* smem) {
+    const VolDesc& V = vols[vol];
+    const LevelDesc& LD = V.lev[level];
+    if (cx >= LD.C[0]) return;
+    if (mode == kFitNum && V.st->done) return;
+    Carve cv{smem, 0};
+    double* red = cv.take<double>(kBlock);
+    const int CyCz = LD.C[1] * LD.C[2];
+    const int lo = LD.sup[0][2 * cx], hi = LD.sup[0][2 * cx + 1];
     const AxisEntry* axx = LD.ax[0];
-    const int* supx = LD.sup[0];
-    const int CyCz = Cy * Cz;
+    const int cols = (CyCz <= kBlock) ? CyCz : kBlock;
+    const int G = kBlock / cols;
     double* lat = V.lat[level & 1];
-    BLOCK_FOR(t)
-    for (int l = t; l < LD.L; l += kBlock) {
-        const int cx = l / CyCz, rest = l - (l / CyCz) * CyCz;
+    for (int p0 = 0; p0 < CyCz; p0 += cols) {
+        BLOCK_FOR(t)
         double acc = 0.0;
-        for (int xx = supx[2 * cx]; xx < supx[2 * cx + 1]; ++xx) {
-            const AxisEntry& e = axx[xx];
-            const float wx = (mode == kFitDen) ? e.q[cx - e.b] : e.p[cx - e.b];
-            acc += (double)wx * (double)V.part[(size_t)xx * V.part_stride + rest];
-        }
-        if (mode == kFitDen) {
-            V.den[l] = acc;
-        } else {
-            const double lres = acc / (V.den[l] + rp.eps);
-            if (mode == kFitNum) {
-                const double nl = lat[l] + lres;
-                lat[l] = nl;
-                V.latf[l] = (float)nl;
+        const int g = t / cols, p = p0 + (t - (t / cols) * cols);
+        if (g < G && p < CyCz) {
+            for (int xx = lo + g; xx < hi; xx += G) {
+                const AxisEntry& e = axx[xx];
+                const float wx = (mode == kFitDen) ? e.q[cx - e.b] : e.p[cx - e.b];
+                acc += (double)wx * (double)V.part[(size_t)xx * V.part_stride + p];
             }
-            V.lresf[l] = (float)lres;
         }
+        red[t] = acc;
+        BLOCK_END
+        BLOCK_SYNC();
+        BLOCK_FOR(t)
+        const int p = p0 + t;
+        if (t < cols && p < CyCz) {
+            double numv = 0.0;
+            for (int g = 0; g < G; ++g) numv += red[g * cols + t];
+            const size_t l = (size_t)cx * CyCz + p;
+            if (mode == kFitDen) {
+                V.den[l] = numv;
+            } else {
+                const double lres = numv / (V.den[l] + rp.eps);
+                if (mode == kFitNum) {
+                    const double nl = lat[l] + lres;
+                    lat[l] = nl;
+                    V.latf[l] = (float)nl;
+                }
+                V.lresf[l] = (float)lres;
+            }
+        }
+        BLOCK_END
+        BLOCK_SYNC();
     }
-    BLOCK_END
 }
 
 // ------------------------------------------------------------------ eval --
@@ -687,8 +805,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
 //   B[y,cz]   = sum_b wy[b] A[by+b,cz]
 //   f(y,z)    = sum_c wz[c] B[y,bz+c];  d = same with Lres = new - old field
 //   logU = imlog - f;  u = exp(old-new) - 1 = expm1(-d)
-// Per slab: sum u, sum u^2 (FP64), min/max of logU; the last slab computes
-// CV = std/mean (ddof 0), the stop flag and the next histogram range.
+// Per slab: spart[x] = (sum u, sum u^2 in FP64, min, max of logU).
 struct EvalLds {
     float* A;
     float* Ar;
@@ -697,7 +814,6 @@ struct EvalLds {
     float* zw;
     int* zb;
     double* red;
-    int* flag;
 };
 
 HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds* o) {
@@ -710,17 +826,16 @@ HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds*
     L.zw = c.take<float>((size_t)4 * Z);
     L.zb = c.take<int>(Z);
     L.red = c.take<double>((size_t)4 * kBlock);
-    L.flag = c.take<int>(1);
     if (o) *o = L;
     return align16(c.off);
 }
 
 HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams rp, int bid,
                   unsigned char* smem) {
+    (void)rp;
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    VolState* st = V.st;
-    if (st->done) return;
+    if (V.st->done) return;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
@@ -754,6 +869,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
     BLOCK_SYNC();
     for (int y0 = 0; y0 < Y; y0 += R) {
         const int ny = (Y - y0 < R) ? Y - y0 : R;
+        const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = pr / Cz, c = pr - (pr / Cz) * Cz;
@@ -771,22 +887,32 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
         BLOCK_FOR(t)
         double s1 = S.red[t], s2 = S.red[kBlock + t];
         double mn = S.red[2 * kBlock + t], mx = S.red[3 * kBlock + t];
-        for (int v = t; v < ny * Z; v += kBlock) {
-            const size_t w = ((size_t)x * Y + y0) * Z + v;
-            if (!V.wmask[w]) continue;
-            const int r = v / Z, z = v - (v / Z) * Z;
-            const float* wz = S.zw + 4 * z;
-            const float* br = S.B + r * Cz + S.zb[z];
-            const float* brr = S.Br + r * Cz + S.zb[z];
-            const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
-            const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
-            const float lg = V.imlog[w] - f;
-            V.logU[w] = lg;
-            const double u = (double)expm1f(-d);
-            s1 += u;
-            s2 += u * u;
-            mn = fmin(mn, (double)lg);
-            mx = fmax(mx, (double)lg);
+        for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
+            uint8_t ms[4];
+            float il[4];
+            UNROLL for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * kBlock + t;
+                const bool ok = v < ny * Z;
+                ms[u] = ok ? V.wmask[base + v] : 0;
+                il[u] = ok ? V.imlog[base + v] : 0.0f;
+            }
+            UNROLL for (int u = 0; u < 4; ++u) {
+                if (!ms[u]) continue;
+                const int v = v0 + u * kBlock + t;
+                const int r = v / Z, z = v - (v / Z) * Z;
+                const float* wz = S.zw + 4 * z;
+                const float* br = S.B + r * Cz + S.zb[z];
+                const float* brr = S.Br + r * Cz + S.zb[z];
+                const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
+                const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
+                const float lg = il[u] - f;
+                V.logU[base + v] = lg;
+                const double uu = (double)expm1f(-d);
+                s1 += uu;
+                s2 += uu * uu;
+                mn = fmin(mn, (double)lg);
+                mx = fmax(mx, (double)lg);
+            }
         }
         S.red[t] = s1;
         S.red[kBlock + t] = s2;
@@ -803,35 +929,6 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
         sp[1] = S.red[kBlock];
         sp[2] = S.red[2 * kBlock];
         sp[3] = S.red[3 * kBlock];
-    }
-    BLOCK_END
-    if (!slab_arrive(&st->cnt[3], (uint32_t)V.W[0], S.flag)) return;
-    BLOCK_FOR(t)
-    if (t == 0) {
-        double s1 = 0.0, s2 = 0.0, mn = INFINITY, mx = -INFINITY;
-        for (int xx = 0; xx < V.W[0]; ++xx) {
-            const double* sp = V.spart + (size_t)xx * 4;
-            s1 += sp[0];
-            s2 += sp[1];
-            mn = fmin(mn, sp[2]);
-            mx = fmax(mx, sp[3]);
-        }
-        const double n = (double)st->n_mask_work;
-        double cvv = 0.0;
-        if (n > 0.0) {
-            const double mu = s1 / n;
-            double var = s2 / n - mu * mu;
-            if (var < 0.0) var = 0.0;
-            cvv = sqrt(var) / (1.0 + mu);
-        }
-        st->iter += 1;
-        st->cv = cvv;
-        st->iters[level] = st->iter;
-        st->lastcv[level] = cvv;
-        // N4.py:301-302: continue while CV > threshold and iterations remain
-        if ((!rp.fixed_iters && !(cvv > rp.thr)) || st->iter >= V.max_iter[level]) st->done = 1;
-        publish_done(V, level, st->done);
-        set_range(st, mn, mx, st->n_mask_work, rp.nbins);
     }
     BLOCK_END
 }
@@ -886,6 +983,7 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
     BLOCK_SYNC();
     for (int y0 = 0; y0 < Y; y0 += R) {
         const int ny = (Y - y0 < R) ? Y - y0 : R;
+        const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = pr / Cz, c = pr - (pr / Cz) * Cz;
@@ -897,20 +995,27 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
         BLOCK_END
         BLOCK_SYNC();
         BLOCK_FOR(t)
-        for (int v = t; v < ny * Z; v += kBlock) {
-            const int r = v / Z, z = v - (v / Z) * Z;
-            const size_t fv = ((size_t)x * Y + y0) * Z + v;
-            const float* wz = S.zw + 4 * z;
-            const float* br = S.B + r * Cz + S.zb[z];
-            const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
-            if (V.out_logbias) V.out_logbias[fv] = f;
-            if (V.out_norm || V.out_bias) {
-                const float ef = expf(f);
-                if (V.out_bias) V.out_bias[fv] = ef;
-                if (V.out_norm) {
-                    const float xin = V.im[fv];
-                    const float xo = (xin <= rp.low_f) ? 0.0f : xin;  // N4.py:285
-                    V.out_norm[fv] = xo / ef;
+        for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
+            float xin[4];
+            UNROLL for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * kBlock + t;
+                xin[u] = (V.out_norm && v < ny * Z) ? V.im[base + v] : 0.0f;
+            }
+            UNROLL for (int u = 0; u < 4; ++u) {
+                const int v = v0 + u * kBlock + t;
+                if (v >= ny * Z) continue;
+                const int r = v / Z, z = v - (v / Z) * Z;
+                const float* wz = S.zw + 4 * z;
+                const float* br = S.B + r * Cz + S.zb[z];
+                const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
+                if (V.out_logbias) V.out_logbias[base + v] = f;
+                if (V.out_norm || V.out_bias) {
+                    const float ef = expf(f);
+                    if (V.out_bias) V.out_bias[base + v] = ef;
+                    if (V.out_norm) {
+                        const float xo = (xin[u] <= rp.low_f) ? 0.0f : xin[u];  // N4.py:285
+                        V.out_norm[base + v] = xo / ef;
+                    }
                 }
             }
         }
@@ -921,20 +1026,22 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
 
 // ------------------------------------------------------------------ flat --
 // 1-D helpers for the stage entry points.  Slab x covers voxels
-// [x*kChunkVox, min((x+1)*kChunkVox, n)) of V.logU / V.wmask.
+// [x*kChunkVox, min((x+1)*kChunkVox, n)) of V.logU / V.wmask.  The
+// reductions write spart[x] (reduced on the host):
+//   minmax (count, count, min, max); sum (sum, count, min, max);
+//   cv (count, sum u, sum u^2, -) with u = exp(a-b) - 1; remap writes out.
 HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a, const float* b,
                   float* out, RunParams rp, int bid, unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
-    double* E = cv.take<double>(kMaxBins);
-    int* flag = cv.take<int>(1);
+    double* E = cv.take<double>(rp.nbins);
     const int64_t n = V.nvox_w;
     const int64_t v0 = (int64_t)sl.x * kChunkVox;
     const int nb = rp.nbins;
-    const double bmin = V.st->bin_min, slope = V.st->slope;
     if (op == kFlatRemap) {
+        const double bmin = V.st->bin_min, slope = V.st->slope;
         BLOCK_FOR(t)
         for (int i = t; i < nb; i += kBlock) E[i] = V.etab[i];
         BLOCK_END
@@ -961,8 +1068,6 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
         BLOCK_END
         return;
     }
-    // slots: minmax (count, -, min, max); sum (sum, count, min, max);
-    //        cv (count, sum u, sum u^2, -)   with u = exp(a-b) - 1
     BLOCK_FOR(t)
     double r0 = 0.0, r1 = 0.0, r2 = (op == kFlatCV) ? 0.0 : INFINITY, r3 = -INFINITY;
     for (int k = t; k < kChunkVox; k += kBlock) {
@@ -982,6 +1087,7 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
                 r1 += 1.0;
             } else {
                 r0 += 1.0;
+                r1 += 1.0;
             }
             r2 = fmin(r2, xv);
             r3 = fmax(r3, xv);
@@ -1005,51 +1111,16 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
         sp[3] = red[3 * kBlock];
     }
     BLOCK_END
-    const uint32_t nslab = (uint32_t)((n + kChunkVox - 1) / kChunkVox);
-    if (!slab_arrive(&V.st->cnt[0], nslab, flag)) return;
-    BLOCK_FOR(t)
-    if (t == 0) {
-        double a0 = 0.0, a1 = 0.0, a2 = (op == kFlatCV) ? 0.0 : INFINITY, a3 = -INFINITY;
-        for (uint32_t xx = 0; xx < nslab; ++xx) {
-            const double* sp = V.spart + (size_t)xx * 4;
-            a0 += sp[0];
-            a1 += sp[1];
-            a2 = (op == kFlatCV) ? a2 + sp[2] : fmin(a2, sp[2]);
-            a3 = fmax(a3, sp[3]);
-        }
-        VolState* st = V.st;
-        if (op == kFlatMinMax) {
-            st->n_mask_work = (uint64_t)a0;
-            set_range(st, a2, a3, (uint64_t)a0, nb);
-        } else if (op == kFlatSum) {
-            st->cv = a0;
-            st->n_mask_work = (uint64_t)a1;
-            st->bin_min = a2;
-            st->bin_max = a3;
-        } else {
-            double cvv = 0.0;
-            if (a0 > 0.0) {
-                const double mu = a1 / a0;
-                double var = a2 / a0 - mu * mu;
-                if (var < 0.0) var = 0.0;
-                cvv = sqrt(var) / (1.0 + mu);
-            }
-            st->cv = cvv;
-            st->n_mask_work = (uint64_t)a0;
-        }
-    }
-    BLOCK_END
 }
 
 // ---------------------------------------------------------------- sizing --
 
-size_t lds_prep() { return align16(4 * kBlock * sizeof(double) + 16); }
-
-size_t lds_hist(int nbins, int P) {
-    const size_t h = align16(nbins * sizeof(unsigned long long)) + 16;
-    const size_t w = (3 * (size_t)P + P / 2) * sizeof(D2) + 64;
-    return align16(h > w ? h : w);
-}
+size_t lds_prep() { return align16(4 * kBlock * sizeof(double)); }
+size_t lds_level() { return align16(4 * kBlock * sizeof(double)); }
+size_t lds_hist(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(uint64_t)); }
+size_t lds_wiener(int P) { return align16((3 * (size_t)P + P / 2) * sizeof(D2)); }
+size_t lds_lattice() { return align16(kBlock * sizeof(double)); }
+size_t lds_flat(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(double)); }
 
 size_t lds_fit(int nbins, const VolDesc& v, int level) {
     const LevelDesc& L = v.lev[level];
@@ -1066,68 +1137,87 @@ size_t lds_recon(const VolDesc& v) {
     return carve_recon(nullptr, v.F[2], rows_per_chunk(v.F[1], v.F[2]), L.C[1], L.C[2], nullptr);
 }
 
-size_t lds_flat() { return align16((4 * kBlock + kMaxBins) * sizeof(double) + 16); }
-
 // --------------------------------------------------------------- launches --
 #ifndef N4_EMULATE
 
+#define SMEM() reinterpret_cast<unsigned char*>(smem_)
+
 __global__ __launch_bounds__(kBlock) void k_prep(const VolDesc* vols, const Slab* slabs, RunParams rp) {
-    HIP_DYNAMIC_SHARED(double, smem);
-    body_prep(vols, slabs, rp, blockIdx.x, reinterpret_cast<unsigned char*>(smem));
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_prep(vols, slabs, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_level(const VolDesc* vols, int level) {
-    body_level(vols, level, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_level(const VolDesc* vols, int level, int prev_iters, RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_level(vols, level, prev_iters, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_hist(const VolDesc* vols, const Slab* slabs, RunParams rp,
-                                                 int32_t* dbg_idx) {
-    HIP_DYNAMIC_SHARED(double, smem);
-    body_hist(vols, slabs, rp, dbg_idx, blockIdx.x, reinterpret_cast<unsigned char*>(smem));
+__global__ __launch_bounds__(kBlock) void k_hist(const VolDesc* vols, const Slab* slabs, int level, int it,
+                                                 RunParams rp, int32_t* dbg_idx) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_hist(vols, slabs, level, it, rp, dbg_idx, blockIdx.x, SMEM());
+}
+
+__global__ __launch_bounds__(kBlock) void k_wiener(const VolDesc* vols, RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_wiener(vols, rp, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_fit(const VolDesc* vols, const Slab* slabs, int level, int mode,
                                                 RunParams rp) {
-    HIP_DYNAMIC_SHARED(double, smem);
-    body_fit(vols, slabs, level, mode, rp, blockIdx.x, reinterpret_cast<unsigned char*>(smem));
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_fit(vols, slabs, level, mode, rp, blockIdx.x, SMEM());
+}
+
+__global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* vols, int level, int mode, RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_lattice(vols, level, mode, rp, blockIdx.x, blockIdx.y, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, int level,
                                                  RunParams rp) {
-    HIP_DYNAMIC_SHARED(double, smem);
-    body_eval(vols, slabs, level, rp, blockIdx.x, reinterpret_cast<unsigned char*>(smem));
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_eval(vols, slabs, level, rp, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_recon(const VolDesc* vols, const Slab* slabs, int level,
                                                   RunParams rp) {
-    HIP_DYNAMIC_SHARED(double, smem);
-    body_recon(vols, slabs, level, rp, blockIdx.x, reinterpret_cast<unsigned char*>(smem));
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_recon(vols, slabs, level, rp, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_flat(const VolDesc* vols, const Slab* slabs, int op,
                                                  const float* a, const float* b, float* out, RunParams rp) {
-    HIP_DYNAMIC_SHARED(double, smem);
-    body_flat(vols, slabs, op, a, b, out, rp, blockIdx.x, reinterpret_cast<unsigned char*>(smem));
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_flat(vols, slabs, op, a, b, out, rp, blockIdx.x, SMEM());
 }
 
 static int last_launch_status() { return (int)hipGetLastError(); }
 
-int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, void* stream) {
+int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int, RunParams rp, void* stream) {
     if (nslab <= 0) return 0;
     hipLaunchKernelGGL(k_prep, dim3(nslab), dim3(kBlock), lds_prep(), (hipStream_t)stream, vols, slabs, rp);
     return last_launch_status();
 }
 
-int launch_level(const VolDesc* vols, int nvol, int level, RunParams, void* stream) {
+int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunParams rp, void* stream) {
     if (nvol <= 0) return 0;
-    hipLaunchKernelGGL(k_level, dim3(nvol), dim3(kBlock), 0, (hipStream_t)stream, vols, level);
+    hipLaunchKernelGGL(k_level, dim3(nvol), dim3(kBlock), lds_level(), (hipStream_t)stream, vols, level, prev_iters,
+                       rp);
     return last_launch_status();
 }
 
-int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, int32_t* dbg_idx, size_t lds,
-                void* stream) {
+int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
+                int32_t* dbg_idx, size_t lds, void* stream) {
     if (nslab <= 0) return 0;
-    hipLaunchKernelGGL(k_hist, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, rp, dbg_idx);
+    hipLaunchKernelGGL(k_hist, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, it, rp,
+                       dbg_idx);
+    return last_launch_status();
+}
+
+int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void* stream) {
+    if (nvol <= 0) return 0;
+    hipLaunchKernelGGL(k_wiener, dim3(nvol), dim3(kBlock), lds, (hipStream_t)stream, vols, rp);
     return last_launch_status();
 }
 
@@ -1135,6 +1225,13 @@ int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int
                void* stream) {
     if (nslab <= 0) return 0;
     hipLaunchKernelGGL(k_fit, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, mode, rp);
+    return last_launch_status();
+}
+
+int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mode, RunParams rp, void* stream) {
+    if (nvol <= 0 || cx_max <= 0) return 0;
+    hipLaunchKernelGGL(k_lattice, dim3(cx_max, nvol), dim3(kBlock), lds_lattice(), (hipStream_t)stream, vols, level,
+                       mode, rp);
     return last_launch_status();
 }
 
@@ -1155,60 +1252,74 @@ int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, R
 int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const float* a, const float* b,
                 float* out, RunParams rp, void* stream) {
     if (nslab <= 0) return 0;
-    hipLaunchKernelGGL(k_flat, dim3(nslab), dim3(kBlock), lds_flat(), (hipStream_t)stream, vols, slabs, op, a,
-                       b, out, rp);
+    hipLaunchKernelGGL(k_flat, dim3(nslab), dim3(kBlock), lds_flat(rp.nbins), (hipStream_t)stream, vols, slabs, op,
+                       a, b, out, rp);
     return last_launch_status();
 }
 
 #else  // N4_EMULATE: host loops over blocks, used only by CPU unit tests
 
-static std::vector<unsigned char>& emu_lds(size_t n) {
+static unsigned char* emu_lds(size_t n) {
     static std::vector<unsigned char> buf;
     if (buf.size() < n + 64) buf.assign(n + 64, 0);
-    return buf;
+    return buf.data();
 }
 
-int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, void*) {
-    auto& b = emu_lds(lds_prep());
-    for (int i = 0; i < nslab; ++i) body_prep(vols, slabs, rp, i, b.data());
+int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int, RunParams rp, void*) {
+    unsigned char* b = emu_lds(lds_prep());
+    for (int i = 0; i < nslab; ++i) body_prep(vols, slabs, rp, i, b);
     return 0;
 }
 
-int launch_level(const VolDesc* vols, int nvol, int level, RunParams, void*) {
-    for (int i = 0; i < nvol; ++i) body_level(vols, level, i);
+int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunParams rp, void*) {
+    unsigned char* b = emu_lds(lds_level());
+    for (int i = 0; i < nvol; ++i) body_level(vols, level, prev_iters, rp, i, b);
     return 0;
 }
 
-int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, int32_t* dbg_idx, size_t lds,
-                void*) {
-    auto& b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_hist(vols, slabs, rp, dbg_idx, i, b.data());
+int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
+                int32_t* dbg_idx, size_t lds, void*) {
+    unsigned char* b = emu_lds(lds);
+    for (int i = 0; i < nslab; ++i) body_hist(vols, slabs, level, it, rp, dbg_idx, i, b);
+    return 0;
+}
+
+int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void*) {
+    unsigned char* b = emu_lds(lds);
+    for (int i = 0; i < nvol; ++i) body_wiener(vols, rp, i, b);
     return 0;
 }
 
 int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int mode, RunParams rp, size_t lds,
                void*) {
-    auto& b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_fit(vols, slabs, level, mode, rp, i, b.data());
+    unsigned char* b = emu_lds(lds);
+    for (int i = 0; i < nslab; ++i) body_fit(vols, slabs, level, mode, rp, i, b);
+    return 0;
+}
+
+int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mode, RunParams rp, void*) {
+    unsigned char* b = emu_lds(lds_lattice());
+    for (int v = 0; v < nvol; ++v)
+        for (int cx = 0; cx < cx_max; ++cx) body_lattice(vols, level, mode, rp, cx, v, b);
     return 0;
 }
 
 int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, void*) {
-    auto& b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_eval(vols, slabs, level, rp, i, b.data());
+    unsigned char* b = emu_lds(lds);
+    for (int i = 0; i < nslab; ++i) body_eval(vols, slabs, level, rp, i, b);
     return 0;
 }
 
 int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, void*) {
-    auto& b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_recon(vols, slabs, level, rp, i, b.data());
+    unsigned char* b = emu_lds(lds);
+    for (int i = 0; i < nslab; ++i) body_recon(vols, slabs, level, rp, i, b);
     return 0;
 }
 
 int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const float* a, const float* b,
                 float* out, RunParams rp, void*) {
-    auto& buf = emu_lds(lds_flat());
-    for (int i = 0; i < nslab; ++i) body_flat(vols, slabs, op, a, b, out, rp, i, buf.data());
+    unsigned char* buf = emu_lds(lds_flat(rp.nbins));
+    for (int i = 0; i < nslab; ++i) body_flat(vols, slabs, op, a, b, out, rp, i, buf);
     return 0;
 }
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "n4_reduce_min", "n4_device_count", "n4_malloc", "n4_free", "n4_memcpy_htod",
     "n4_memcpy_dtoh", "n4_memset", "n4_synchronize", "n4_profile_enable", "n4_profile_read",
 ]
-KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon"]
+KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice"]
 
 
 class Params(C.Structure):

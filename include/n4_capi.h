@@ -181,13 +181,15 @@ n4_status n4_reduce_min(n4_handle *h, uint64_t n, const float *d_in, float *out)
 /* Kernels of the correction, for n4_profile_*. */
 typedef enum n4_kernel_id {
     N4_K_PREP = 0,    /* threshold + log + shrink (full resolution) */
-    N4_K_LEVEL = 1,   /* level setup / lattice refinement */
-    N4_K_HIST = 2,    /* fractional histogram + Wiener tail */
-    N4_K_FIT_DEN = 3, /* fit denominator (once per level) */
-    N4_K_FIT = 4,     /* remap + residual + fit numerator + lattice tail */
-    N4_K_EVAL = 5,    /* field + convergence + logU update */
+    N4_K_LEVEL = 1,   /* level setup / lattice refinement / commit */
+    N4_K_HIST = 2,    /* convergence decision + fractional histogram */
+    N4_K_FIT_DEN = 3, /* fit denominator partials (once per level) */
+    N4_K_FIT = 4,     /* remap + residual + fit numerator partials */
+    N4_K_EVAL = 5,    /* field + convergence partials + logU update */
     N4_K_RECON = 6,   /* full-resolution field + normalisation */
-    N4_K_COUNT = 7
+    N4_K_WIENER = 7,  /* Wiener deconvolution + E table (one block/volume) */
+    N4_K_LATTICE = 8, /* mode-x contraction + lattice update */
+    N4_K_COUNT = 9
 } n4_kernel_id;
 /* Time every launch of the kernels in kernel_mask (bit = n4_kernel_id) with
  * HIP events on the launch stream; clears the accumulators. 0 disables. */

@@ -63,14 +63,15 @@ def test_n4_golden_trajectories(gpu, golden, case):
 
 
 def test_n4_golden_ill_conditioned(gpu, golden):
-    """All-foreground mask with scattered background voxels: the oracle
-    itself moves by up to ~1e-4 under 1-ulp input changes, so the bar is
-    twice that measured sensitivity."""
+    """All-foreground mask with scattered background voxels (729 masked
+    voxels): the oracle itself moves by up to ~1.4e-4 under 1-ulp input
+    changes (the histogram's top-edge rule, N4.py:25-27, is discontinuous),
+    so the bar is three times the largest change seen over six trials."""
     case = 2
     im, m = golden[f"n4_{case}_im"], golden[f"n4_{case}_mask"]
     kw = dict(levels=2, max_iter=[5, 5], conv_threshold=0.001)
-    sens = PC.sensitivity(im, m, **kw)
-    PC.check_n4_vs_oracle(gpu, im, m, 2, [5, 5], 0.001, tol=max(PC.LOGBIAS_TOL, 2 * sens))
+    sens = PC.sensitivity(im, m, n_trials=6, **kw)
+    PC.check_n4_vs_oracle(gpu, im, m, 2, [5, 5], 0.001, tol=max(PC.LOGBIAS_TOL, 3 * sens))
 
 
 def test_n4_c0_reference_case(gpu):
