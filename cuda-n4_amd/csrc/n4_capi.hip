@@ -3,8 +3,9 @@
 // A call plans the batch on the host (working grids, per-level control
 // point counts, FP64 axis tables, slab lists, arena offsets), uploads the
 // plan with one copy, then enqueues the level loop on the caller's stream:
-//   prep -> for each level: level setup, fit(DEN), iterations x
-//   {hist(+Wiener tail), fit(NUM, + lattice tail), eval(+CV tail)} -> recon.
+//   prep -> for each level: level setup, fit(DEN) + lattice(DEN),
+//   iterations x {hist, wiener, fit, lattice, eval} -> recon.
+// Batches are split into volume groups that run on their own streams.
 // Nothing is allocated inside the loop: the handle's arena grows only when a
 // larger batch arrives.  Under N4_EMULATE the same driver runs the kernel
 // bodies on the host (CPU unit tests of the kernel logic only).
@@ -80,6 +81,7 @@ static void dev_free(void* p) { free(p); }
 
 // ---------------------------------------------------------------- handle --
 static const int kPipelineSlots = 4;
+static const int kMaxGroups = 8;
 
 struct n4_handle {
     int device = 0;
@@ -90,8 +92,10 @@ struct n4_handle {
     int32_t* dev_flags = nullptr;        // the same memory as seen by the device
     int flag_cap = 0;
 #ifndef N4_EMULATE
-    hipEvent_t flag_ev[kPipelineSlots];
+    hipEvent_t flag_ev[kMaxGroups][kPipelineSlots];
     bool flag_ev_init = false;
+    hipStream_t gstream[kMaxGroups] = {};  // streams of batch groups 1..
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxGroups] = {};
     std::vector<hipEvent_t> prof_ev;       // event pool for kernel timing
     std::vector<int> prof_pending;         // kernel id per recorded pair
 #endif
@@ -221,15 +225,30 @@ struct Plan {
     std::vector<VolDesc> desc;  // host copies (pointers into the arena)
     size_t upload_bytes = 0;    // [0, upload_bytes) of the arena comes from staging
     size_t total_bytes = 0;
-    size_t off_desc = 0, off_wslab = 0, off_fslab = 0, off_state = 0;
-    int n_wslab = 0, n_fslab = 0;
+    size_t off_desc = 0, off_wslab = 0, off_fslab = 0, off_pslab = 0, off_state = 0, off_tw = 0;
+    int n_wslab = 0, n_fslab = 0, n_pslab = 0;
     std::vector<int> max_iter;  // per level (max over volumes)
+    std::vector<int> recon_tiles;  // recon tiles per volume
     size_t lds_fit_max[kMaxLevels] = {0}, lds_eval_max[kMaxLevels] = {0};
     int cx_max[kMaxLevels] = {0};
     size_t lds_hist = 0, lds_wiener = 0, lds_recon = 0;
 };
 
 static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
+
+// Magic multiplier for floor(v / d) (see fdiv in the kernels).
+static uint64_t magic_div(uint32_t d) { return ((uint64_t)1 << kDivShift) / d + 1; }
+
+// Working voxels per prep tile and full-resolution voxels per recon tile:
+// enough workgroups to fill the device even for one volume.
+static const int kPrepTileVox = 1024;
+static const int kReconTileVox = 8192;
+
+static int rows_for(int rows, int cols, int target) {
+    int r = target / (cols > 0 ? cols : 1);
+    if (r < 1) r = 1;
+    return r < rows ? r : rows;
+}
 
 // Builds the plan and fills h->staging with the upload region.  Pointers in
 // the descriptors are arena-relative until `rebase` adds the arena base.
@@ -248,9 +267,10 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         int W[3];
         int C[kMaxLevels][3];
         size_t Lmax, part_stride;
+        int prep_rows, prep_tiles_x, recon_rows, recon_tiles_x;
     };
     std::vector<VInfo> vi(nv);
-    int nws = 0, nfs = 0;
+    int nws = 0, nfs = 0, nps = 0;
     for (int v = 0; v < nv; ++v) {
         VInfo& I = vi[v];
         for (int d = 0; d < 3; ++d) I.W[d] = (in[v].F[d] + in[v].s - 1) / in[v].s;
@@ -259,17 +279,27 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         const int* Cf = I.C[L - 1];
         I.Lmax = (size_t)Cf[0] * Cf[1] * Cf[2];
         I.part_stride = (size_t)Cf[1] * Cf[2];
+        I.prep_rows = rows_for(I.W[1], I.W[2], kPrepTileVox);
+        I.prep_tiles_x = (I.W[1] + I.prep_rows - 1) / I.prep_rows;
+        I.recon_rows = rows_for(in[v].F[1], in[v].F[2], kReconTileVox);
+        I.recon_tiles_x = (in[v].F[1] + I.recon_rows - 1) / I.recon_rows;
         nws += I.W[0];
-        nfs += in[v].F[0];
+        nps += I.W[0] * I.prep_tiles_x;
+        nfs += in[v].F[0] * I.recon_tiles_x;
     }
     P->n_wslab = nws;
+    P->n_pslab = nps;
+    P->recon_tiles.assign(nv, 0);
+    for (int v = 0; v < nv; ++v) P->recon_tiles[v] = need_tables_full ? in[v].F[0] * vi[v].recon_tiles_x : 0;
     P->n_fslab = need_tables_full ? nfs : 0;
 
     Layout lay;
     P->off_desc = lay.take(sizeof(VolDesc) * nv);
     P->off_wslab = lay.take(sizeof(Slab) * (nws > 0 ? nws : 1));
     P->off_fslab = lay.take(sizeof(Slab) * (P->n_fslab > 0 ? P->n_fslab : 1));
+    P->off_pslab = lay.take(sizeof(Slab) * (nps > 0 ? nps : 1));
     P->off_state = lay.take(sizeof(VolState) * nv);
+    P->off_tw = lay.take(sizeof(D2) * P->rp.P);
     // tables
     std::vector<size_t> off_ax(nv * kMaxLevels * 3), off_sup(nv * kMaxLevels * 3), off_fax(nv * 3);
     for (int v = 0; v < nv; ++v)
@@ -285,7 +315,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
 
     // per-volume working buffers
     struct Offs {
-        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, hist, spart, etab;
+        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, hist, spart, etab, ppart;
     };
     std::vector<Offs> off(nv);
     const int nb = P->rp.nbins;
@@ -304,6 +334,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
         o.hist = lay.take(sizeof(uint64_t) * nb);
         o.spart = lay.take(sizeof(double) * 4 * I.W[0]);
+        o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
         o.etab = lay.take(sizeof(double) * nb);
     }
     P->total_bytes = lay.take(0);
@@ -318,7 +349,16 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     P->desc.assign(nv, VolDesc());
     Slab* ws = (Slab*)(hb + P->off_wslab);
     Slab* fs = (Slab*)(hb + P->off_fslab);
-    int iws = 0, ifs = 0;
+    Slab* ps = (Slab*)(hb + P->off_pslab);
+    int iws = 0, ifs = 0, ips = 0;
+    {
+        D2* tw = (D2*)(hb + P->off_tw);
+        for (int m = 0; m < P->rp.P; ++m) {
+            const double ang = 2.0 * (double)m / (double)P->rp.P;  // in units of pi
+            tw[m] = D2{cos(M_PI * ang), -sin(M_PI * ang)};
+        }
+        P->rp.twiddle = (const D2*)(base + P->off_tw);
+    }
     for (int v = 0; v < nv; ++v) {
         const VInfo& I = vi[v];
         VolDesc D;
@@ -348,6 +388,11 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.part = (float*)(base + o.part);
         D.hist = (uint64_t*)(base + o.hist);
         D.spart = (double*)(base + o.spart);
+        D.ppart = (double*)(base + o.ppart);
+        D.prep_rows = I.prep_rows;
+        D.prep_tiles_x = I.prep_tiles_x;
+        D.mW2 = magic_div((uint32_t)I.W[2]);
+        D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
         D.st = (VolState*)(base + P->off_state) + v;
         D.done_flag = (h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
@@ -365,6 +410,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
                 LD.sup[d] = (const int32_t*)(base + os);
             }
             LD.L = LD.C[0] * LD.C[1] * LD.C[2];
+            LD.mCz = magic_div((uint32_t)LD.C[2]);
         }
         if (need_tables_full) {
             const int* Cf = I.C[L - 1];
@@ -376,9 +422,15 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             }
         }
         P->desc[v] = D;
-        for (int x = 0; x < I.W[0]; ++x) ws[iws++] = Slab{v, x};
+        for (int x = 0; x < I.W[0]; ++x) {
+            ws[iws++] = Slab{v, x, 0, I.W[1]};
+            for (int y0 = 0; y0 < I.W[1]; y0 += I.prep_rows)
+                ps[ips++] = Slab{v, x, y0, std::min(I.prep_rows, I.W[1] - y0)};
+        }
         if (need_tables_full)
-            for (int x = 0; x < in[v].F[0]; ++x) fs[ifs++] = Slab{v, x};
+            for (int x = 0; x < in[v].F[0]; ++x)
+                for (int y0 = 0; y0 < in[v].F[1]; y0 += I.recon_rows)
+                    fs[ifs++] = Slab{v, x, y0, std::min(I.recon_rows, in[v].F[1] - y0)};
         for (int l = 0; l < L; ++l) {
             P->lds_fit_max[l] = std::max(P->lds_fit_max[l], lds_fit(nb, D, l));
             P->lds_eval_max[l] = std::max(P->lds_eval_max[l], lds_eval(D, l));
@@ -413,6 +465,9 @@ static inline Slab* dev_wslab(n4_handle* h, const Plan& P) {
 }
 static inline Slab* dev_fslab(n4_handle* h, const Plan& P) {
     return (Slab*)((unsigned char*)h->arena + P.off_fslab);
+}
+static inline Slab* dev_pslab(n4_handle* h, const Plan& P) {
+    return (Slab*)((unsigned char*)h->arena + P.off_pslab);
 }
 static inline VolState* dev_state(n4_handle* h, const Plan& P) {
     return (VolState*)((unsigned char*)h->arena + P.off_state);
@@ -511,12 +566,6 @@ static void prof_collect(n4_handle* h) {
 #endif
 }
 
-#define TIMED(id, expr)                          \
-    do {                                         \
-        ProfScope ps_(h, (id), stream);          \
-        LAUNCH(expr);                            \
-    } while (0)
-
 static n4_status ensure_flags(n4_handle* h, int nvol) {
     if (nvol <= h->flag_cap) return N4_OK;
 #ifndef N4_EMULATE
@@ -526,7 +575,11 @@ static n4_status ensure_flags(n4_handle* h, int nvol) {
     N4_CHECK_HIP(hipHostMalloc((void**)&h->pinned_flags, sizeof(int32_t) * nvol, hipHostMallocDefault));
     N4_CHECK_HIP(hipHostGetDevicePointer((void**)&h->dev_flags, h->pinned_flags, 0));
     if (!h->flag_ev_init) {
-        for (int k = 0; k < kFlagSlots; ++k) N4_CHECK_HIP(hipEventCreateWithFlags(&h->flag_ev[k], hipEventDisableTiming));
+        for (int g = 0; g < kMaxGroups; ++g)
+            for (int k = 0; k < kFlagSlots; ++k)
+                N4_CHECK_HIP(hipEventCreateWithFlags(&h->flag_ev[g][k], hipEventDisableTiming));
+        N4_CHECK_HIP(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+        for (int g = 0; g < kMaxGroups; ++g) N4_CHECK_HIP(hipEventCreateWithFlags(&h->join_ev[g], hipEventDisableTiming));
         h->flag_ev_init = true;
     }
 #else
@@ -538,47 +591,117 @@ static n4_status ensure_flags(n4_handle* h, int nvol) {
     return N4_OK;
 }
 
-// Records the end of iteration `it` and inspects the flags once iteration
-// it - kPipeline has completed.  Flags only move from 0 to level+1 within a
-// level, so reading a newer state than that iteration's is harmless.
-static n4_status poll_done(n4_handle* h, int nvol, int level, int it, void* stream, bool* all_done) {
+// Records the end of iteration `it` of group g and inspects the flags of
+// volumes [v0, v1) once iteration it - kPipeline has completed.  Flags only
+// move from 0 to level+1 within a level, so reading a newer state than that
+// iteration's is harmless.
+static n4_status poll_done(n4_handle* h, int g, int v0, int v1, int level, int it, void* stream, bool* all_done) {
     *all_done = false;
 #ifndef N4_EMULATE
-    N4_CHECK_HIP(hipEventRecord(h->flag_ev[it % kFlagSlots], (hipStream_t)stream));
+    N4_CHECK_HIP(hipEventRecord(h->flag_ev[g][it % kFlagSlots], (hipStream_t)stream));
     const int old = it - kPipeline;
     if (old < 0) return N4_OK;
-    N4_CHECK_HIP(hipEventSynchronize(h->flag_ev[old % kFlagSlots]));
+    N4_CHECK_HIP(hipEventSynchronize(h->flag_ev[g][old % kFlagSlots]));
 #else
+    (void)g;
     (void)stream;
     (void)it;
 #endif
     const volatile int32_t* f = h->pinned_flags;
     bool all = true;
-    for (int v = 0; v < nvol && all; ++v) all = f[v] == level + 1;
+    for (int v = v0; v < v1 && all; ++v) all = f[v] == level + 1;
     *all_done = all;
     return N4_OK;
 }
 
-static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, n4_report* reports,
+// Volume groups of a batch: each group runs its own level/iteration chain
+// on its own stream (group 0 on the caller's stream), so the small
+// per-volume kernels of one group overlap the slab kernels of the others and
+// a group whose volumes converge stops without waiting for the rest.
+// Volumes are dealt to groups largest first (working voxels), each to the
+// group with the least work so far.
+static int group_count(int nvol) {
+    int g = nvol >= 16 ? 4 : (nvol >= 4 ? 2 : 1);
+    if (const char* e = getenv("N4_GROUPS")) {
+        const int v = atoi(e);
+        if (v >= 1) g = v;
+    }
+    if (g > kMaxGroups) g = kMaxGroups;
+    if (g > nvol) g = nvol;
+    return g < 1 ? 1 : g;
+}
+
+struct GroupRun {
+    int v0 = 0, v1 = 0;    // volumes
+    int ws0 = 0, ws1 = 0;  // working slabs
+    int fs0 = 0, fs1 = 0;  // recon tiles
+    void* stream = nullptr;
+    int level = 0, it = 0, prev_iters = 0;
+    bool started = false, finished = false;
+};
+
+#define GLAUNCH(id, expr)                  \
+    do {                                   \
+        ProfScope ps_(h, (id), gs);        \
+        LAUNCH(expr);                      \
+    } while (0)
+
+static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, const n4_params* p, n4_report* reports,
                            void* stream) {
     g_err.clear();
     n4_status s = check_params(p);
     if (s != N4_OK) return s;
-    for (const VolIn& v : in) {
+    for (const VolIn& v : in_user) {
         s = check_volume(v);
         if (s != N4_OK) return s;
     }
-    if (in.empty()) return N4_OK;
+    if (in_user.empty()) return N4_OK;
+    const int nv = (int)in_user.size();
+    const int G = group_count(nv);
+    // order volumes by group (largest first onto the least loaded group)
+    std::vector<int> order(nv), gid(nv, 0);
+    {
+        std::vector<std::pair<double, int>> sz(nv);
+        for (int v = 0; v < nv; ++v) {
+            const VolIn& a = in_user[v];
+            double w = 1.0;
+            for (int d = 0; d < 3; ++d) w *= (double)((a.F[d] + a.s - 1) / a.s);
+            sz[v] = {-w, v};
+        }
+        std::sort(sz.begin(), sz.end());
+        std::vector<double> load(G, 0.0);
+        for (auto& e : sz) {
+            const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            gid[e.second] = g;
+            load[g] += -e.first;
+        }
+        int k = 0;
+        for (int g = 0; g < G; ++g)
+            for (int v = 0; v < nv; ++v)
+                if (gid[v] == g) order[k++] = v;
+    }
+    std::vector<VolIn> in(nv);
+    for (int k = 0; k < nv; ++k) in[k] = in_user[order[k]];
 #ifndef N4_EMULATE
     N4_CHECK_HIP(hipSetDevice(h->device));
-    hipEvent_t e0, e1;
-    N4_CHECK_HIP(hipEventCreate(&e0));
-    N4_CHECK_HIP(hipEventCreate(&e1));
-    N4_CHECK_HIP(hipEventRecord(e0, (hipStream_t)stream));
 #endif
-    s = ensure_flags(h, (int)in.size());
+    s = ensure_flags(h, nv);
     if (s != N4_OK) return s;
-    for (size_t v = 0; v < in.size(); ++v) h->pinned_flags[v] = 0;
+    for (int v = 0; v < nv; ++v) h->pinned_flags[v] = 0;
+#ifndef N4_EMULATE
+    for (int g = 1; g < G; ++g)
+        if (!h->gstream[g]) N4_CHECK_HIP(hipStreamCreateWithFlags(&h->gstream[g], hipStreamNonBlocking));
+    struct EvPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~EvPair() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    N4_CHECK_HIP(hipEventCreate(&ev.a));
+    N4_CHECK_HIP(hipEventCreate(&ev.b));
+    N4_CHECK_HIP(hipEventRecord(ev.a, (hipStream_t)stream));
+#endif
     Plan P;
     s = build_plan(h, in, p, &P);
     if (s != N4_OK) return s;
@@ -589,62 +712,122 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in, const n4_
     Slab* wsl = dev_wslab(h, P);
     Slab* fsl = dev_fslab(h, P);
     const RunParams rp = P.rp;
-    TIMED(N4_K_PREP, launch_prep(dd, wsl, P.n_wslab, 0, rp, stream));
-    int prev_iters = 0;
-    for (int l = 0; l < P.nlevels; ++l) {
-        TIMED(N4_K_LEVEL, launch_level(dd, P.nvol, l, prev_iters, rp, stream));
-        TIMED(N4_K_FIT_DEN, launch_fit(dd, wsl, P.n_wslab, l, kFitDen, rp, P.lds_fit_max[l], stream));
-        TIMED(N4_K_LATTICE, launch_lattice(dd, P.nvol, P.cx_max[l], l, kFitDen, rp, stream));
-        int it = 0;
-        for (; it < P.max_iter[l]; ++it) {
-            TIMED(N4_K_HIST, launch_hist(dd, wsl, P.n_wslab, l, it, rp, nullptr, P.lds_hist, stream));
-            TIMED(N4_K_WIENER, launch_wiener(dd, P.nvol, rp, P.lds_wiener, stream));
-            TIMED(N4_K_FIT, launch_fit(dd, wsl, P.n_wslab, l, kFitNum, rp, P.lds_fit_max[l], stream));
-            TIMED(N4_K_LATTICE, launch_lattice(dd, P.nvol, P.cx_max[l], l, kFitNum, rp, stream));
-            TIMED(N4_K_EVAL, launch_eval(dd, wsl, P.n_wslab, l, rp, P.lds_eval_max[l], stream));
-            if (it + 1 == P.max_iter[l]) continue;
-            bool all_done = false;
-            s = poll_done(h, P.nvol, l, it, stream, &all_done);
-            if (s != N4_OK) return s;
-            if (all_done) {
-                ++it;
-                break;
-            }
-        }
-        prev_iters = it;
+    {
+        void* gs = stream;
+        GLAUNCH(N4_K_PREP, launch_prep(dd, dev_pslab(h, P), P.n_pslab, 0, rp, gs));
     }
-    TIMED(N4_K_LEVEL, launch_level(dd, P.nvol, P.nlevels, prev_iters, rp, stream));
-    TIMED(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, stream));
-    for (int v = 0; v < P.nvol; ++v)
-        if (in[v].out_lattice) {
-            const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
-            D2D(in[v].out_lattice, P.desc[v].latf, sizeof(float) * LD.L, stream);
+    // group ranges (volumes, working slabs and recon tiles are contiguous)
+    std::vector<GroupRun> gr(G);
+    {
+        int v = 0, ws = 0, fs = 0;
+        for (int g = 0; g < G; ++g) {
+            GroupRun& R = gr[g];
+            R.v0 = v;
+            R.ws0 = ws;
+            R.fs0 = fs;
+            while (v < nv && gid[order[v]] == g) {
+                ws += P.desc[v].W[0];
+                fs += P.recon_tiles[v];
+                ++v;
+            }
+            R.v1 = v;
+            R.ws1 = ws;
+            R.fs1 = fs;
+#ifndef N4_EMULATE
+            R.stream = g == 0 ? stream : (void*)h->gstream[g];
+#else
+            R.stream = stream;
+#endif
         }
-    std::vector<VolState> st(P.nvol);
-    D2H(st.data(), dev_state(h, P), sizeof(VolState) * P.nvol, stream);
+    }
+#ifndef N4_EMULATE
+    if (G > 1) {
+        N4_CHECK_HIP(hipEventRecord(h->fork_ev, (hipStream_t)stream));
+        for (int g = 1; g < G; ++g) N4_CHECK_HIP(hipStreamWaitEvent(h->gstream[g], h->fork_ev, 0));
+    }
+#endif
+    // round-robin over groups, one iteration (or level transition) at a time
+    int remaining = G;
+    while (remaining > 0) {
+        for (int g = 0; g < G; ++g) {
+            GroupRun& R = gr[g];
+            if (R.finished) continue;
+            void* gs = R.stream;
+            const int nvg = R.v1 - R.v0, nws = R.ws1 - R.ws0;
+            const VolDesc* gv = dd + R.v0;
+            const Slab* gsl = wsl + R.ws0;
+            const int l = R.level;
+            if (!R.started) {
+                GLAUNCH(N4_K_LEVEL, launch_level(gv, nvg, l, R.prev_iters, rp, gs));
+                GLAUNCH(N4_K_FIT_DEN, launch_fit(dd, gsl, nws, l, kFitDen, rp, P.lds_fit_max[l], gs));
+                GLAUNCH(N4_K_LATTICE, launch_lattice(gv, nvg, P.cx_max[l], l, kFitDen, rp, gs));
+                R.started = true;
+            }
+            bool level_end = (P.max_iter[l] <= 0);
+            if (!level_end) {
+                const int it = R.it;
+                GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, l, it, rp, nullptr, P.lds_hist, gs));
+                GLAUNCH(N4_K_WIENER, launch_wiener(gv, nvg, rp, P.lds_wiener, gs));
+                GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, l, kFitNum, rp, P.lds_fit_max[l], gs));
+                GLAUNCH(N4_K_LATTICE, launch_lattice(gv, nvg, P.cx_max[l], l, kFitNum, rp, gs));
+                GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, l, rp, P.lds_eval_max[l], gs));
+                R.it = it + 1;
+                if (R.it >= P.max_iter[l]) {
+                    level_end = true;
+                } else {
+                    bool all_done = false;
+                    s = poll_done(h, g, R.v0, R.v1, l, it, gs, &all_done);
+                    if (s != N4_OK) return s;
+                    level_end = all_done;
+                }
+            }
+            if (!level_end) continue;
+            R.prev_iters = R.it;
+            R.it = 0;
+            R.started = false;
+            R.level = l + 1;
+            if (R.level < P.nlevels) continue;
+            GLAUNCH(N4_K_LEVEL, launch_level(gv, nvg, P.nlevels, R.prev_iters, rp, gs));
+            GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon, gs));
+            for (int v = R.v0; v < R.v1; ++v)
+                if (in[v].out_lattice) {
+                    const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
+                    D2D(in[v].out_lattice, P.desc[v].latf, sizeof(float) * LD.L, gs);
+                }
+#ifndef N4_EMULATE
+            if (g > 0) {
+                N4_CHECK_HIP(hipEventRecord(h->join_ev[g], (hipStream_t)gs));
+                N4_CHECK_HIP(hipStreamWaitEvent((hipStream_t)stream, h->join_ev[g], 0));
+            }
+#endif
+            R.finished = true;
+            --remaining;
+        }
+    }
+    std::vector<VolState> st(nv);
+    D2H(st.data(), dev_state(h, P), sizeof(VolState) * nv, stream);
     float ms = 0.0f;
 #ifndef N4_EMULATE
-    N4_CHECK_HIP(hipEventRecord(e1, (hipStream_t)stream));
-    N4_CHECK_HIP(hipEventSynchronize(e1));
-    N4_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    N4_CHECK_HIP(hipEventRecord(ev.b, (hipStream_t)stream));
+    N4_CHECK_HIP(hipEventSynchronize(ev.b));
+    N4_CHECK_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
     N4_CHECK_HIP(hipGetLastError());
 #endif
     prof_collect(h);
     if (reports)
-        for (int v = 0; v < P.nvol; ++v) {
+        for (int k = 0; k < nv; ++k) {
+            const int v = order[k];
             n4_report& r = reports[v];
             memset(&r, 0, sizeof r);
             for (int l = 0; l < P.nlevels; ++l) {
-                r.iterations[l] = (uint32_t)st[v].iters[l];
-                r.last_convergence[l] = st[v].lastcv[l];
+                r.iterations[l] = (uint32_t)st[k].iters[l];
+                r.last_convergence[l] = st[k].lastcv[l];
             }
-            r.n_mask_full = st[v].n_mask_full;
-            r.n_mask_work = st[v].n_mask_work;
+            r.n_mask_full = st[k].n_mask_full;
+            r.n_mask_work = st[k].n_mask_work;
             for (int d = 0; d < 3; ++d) {
-                r.work_dims[d] = (uint32_t)P.desc[v].W[d];
-                r.final_ncpt[d] = (uint32_t)P.desc[v].lev[P.nlevels - 1].C[d];
+                r.work_dims[d] = (uint32_t)P.desc[k].W[d];
+                r.final_ncpt[d] = (uint32_t)P.desc[k].lev[P.nlevels - 1].C[d];
             }
             r.elapsed_ms = ms;
         }
@@ -739,8 +922,14 @@ n4_status n4_destroy(n4_handle* h) {
     dev_free(h->arena);
 #ifndef N4_EMULATE
     if (h->pinned_flags) (void)hipHostFree(h->pinned_flags);
-    if (h->flag_ev_init)
-        for (int k = 0; k < kFlagSlots; ++k) (void)hipEventDestroy(h->flag_ev[k]);
+    if (h->flag_ev_init) {
+        for (int g = 0; g < kMaxGroups; ++g)
+            for (int k = 0; k < kFlagSlots; ++k) (void)hipEventDestroy(h->flag_ev[g][k]);
+        (void)hipEventDestroy(h->fork_ev);
+        for (int g = 0; g < kMaxGroups; ++g) (void)hipEventDestroy(h->join_ev[g]);
+    }
+    for (int g = 0; g < kMaxGroups; ++g)
+        if (h->gstream[g]) (void)hipStreamDestroy(h->gstream[g]);
 #else
     free(h->pinned_flags);
 #endif

@@ -37,6 +37,7 @@ struct LevelDesc {
     int32_t L;               // C0*C1*C2
     const AxisEntry* ax[3];  // W[d] entries (working grid)
     const int32_t* sup[3];   // 2*C[d]: [lo, hi) positions whose support holds c
+    uint64_t mCz;            // magic divisor for C[2]
 };
 
 struct VolState {
@@ -85,9 +86,19 @@ struct VolDesc {
     int32_t nlevels;
     int32_t max_iter[kMaxLevels];
     int32_t part_stride;       // floats per slab in part (max Cy*Cz)
+    int32_t prep_rows;         // working rows per prep tile
+    int32_t prep_tiles_x;      // prep tiles per working slab
+    double* ppart;             // prep partials [W0 * prep_tiles_x][4]
+    uint64_t mW2;              // magic divisor for the working z extent
+    uint64_t mF2;              // magic divisor for the full-resolution z extent
+};
+
+struct D2 {
+    double x, y;
 };
 
 struct RunParams {
+    const D2* twiddle;  // exp(-2 pi i k / P), k < P/2 (host-computed)
     int32_t nbins;
     int32_t P;        // padded FFT length (power of two)
     int32_t log2P;
@@ -101,10 +112,18 @@ struct RunParams {
     int32_t fixed_iters;
 };
 
+// One workgroup's tile: rows [y0, y0+ny) of slab x of volume vol.  The
+// working-grid iteration kernels always take whole slabs (y0 = 0).
 struct Slab {
     int32_t vol;
     int32_t x;
+    int32_t y0;
+    int32_t ny;
 };
+
+// Exact floor(v / d) for v * d < 2^40 as one 64-bit multiply and shift:
+// m = floor(2^40 / d) + 1 (computed on the host).
+constexpr int kDivShift = 40;
 
 enum FitMode : int32_t { kFitNum = 0, kFitDen = 1, kFitField = 2 };
 

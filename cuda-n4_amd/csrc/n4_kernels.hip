@@ -56,10 +56,6 @@ namespace n4 {
 
 // ---------------------------------------------------------------- helpers --
 
-struct D2 {
-    double x, y;
-};
-
 HD D2 cmul(D2 a, D2 b) { return D2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
 HD void lds_add_u64(unsigned long long* p, unsigned long long v) {
@@ -88,6 +84,10 @@ HD void sincospi_d(double x, double* s, double* c) {
 }
 
 HD size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+
+// floor(v / d) with the host-computed magic m = floor(2^40/d) + 1
+// (exact while v * d < 2^40).
+HD uint32_t fdiv(uint32_t v, uint64_t m) { return (uint32_t)(((uint64_t)v * m) >> kDivShift); }
 
 // Bump allocator over the block's dynamic LDS; with base == nullptr it only
 // measures (host-side sizing uses the same code as the device carve).
@@ -137,12 +137,12 @@ HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
     }
 }
 
-// Reduces the volume's per-slab partials spart[x] = (s0, s1, min, max).
-HD void reduce_slab_partials(const VolDesc& V, int nslab, double* red) {
+// Reduces n partial quadruples part[i] = (s0, s1, min, max).
+HD void reduce_partials(const double* part, int nslab, double* red) {
     BLOCK_FOR(t)
     double r0 = 0.0, r1 = 0.0, r2 = INFINITY, r3 = -INFINITY;
     for (int xx = t; xx < nslab; xx += kBlock) {
-        const double* sp = V.spart + (size_t)xx * 4;
+        const double* sp = part + (size_t)xx * 4;
         r0 += sp[0];
         r1 += sp[1];
         r2 = fmin(r2, sp[2]);
@@ -240,8 +240,9 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsign
     BLOCK_FOR(t)
     double nfull = 0.0, nwork = 0.0;
     double mn = INFINITY, mx = -INFINITY;
-    for (int v = t; v < W1 * W2; v += kBlock) {
-        const int J = v / W2, K = v - (v / W2) * W2;
+    for (int v = t; v < sl.ny * W2; v += kBlock) {
+        const int jr = (int)fdiv((uint32_t)v, V.mW2);
+        const int J = sl.y0 + jr, K = v - jr * W2;
         int nm = 0, nb = 0;
         float acc = 0.0f;
         prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
@@ -266,7 +267,7 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsign
     tree_reduce4(red, 0, 0, 1, 2);
     BLOCK_FOR(t)
     if (t == 0) {
-        double* sp = V.spart + (size_t)x * 4;
+        double* sp = V.ppart + ((size_t)x * V.prep_tiles_x + sl.y0 / V.prep_rows) * 4;
         sp[0] = red[0];
         sp[1] = red[kBlock];
         sp[2] = red[2 * kBlock];
@@ -310,7 +311,22 @@ HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp,
     VolState* st = V.st;
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
-    reduce_slab_partials(V, V.W[0], red);
+    if (level == 0)
+        reduce_partials(V.ppart, V.W[0] * V.prep_tiles_x, red);
+    else
+        reduce_partials(V.spart, V.W[0], red);
+    if (level == 0) {
+        // seed the slab partials with the initial logU range for iteration 0
+        BLOCK_FOR(t)
+        for (int xx = t; xx < V.W[0]; xx += kBlock) {
+            double* sp = V.spart + (size_t)xx * 4;
+            sp[0] = 0.0;
+            sp[1] = 0.0;
+            sp[2] = xx == 0 ? red[2 * kBlock] : INFINITY;
+            sp[3] = xx == 0 ? red[3 * kBlock] : -INFINITY;
+        }
+        BLOCK_END
+    }
     BLOCK_FOR(t)
     if (t == 0) {
         if (level == 0) {
@@ -409,7 +425,7 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, Run
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
     unsigned long long* h = cv.take<unsigned long long>(nb);
-    reduce_slab_partials(V, V.W[0], red);
+    reduce_partials(V.spart, V.W[0], red);
     const double n = (double)st->n_mask_work;
     const double mn = red[2 * kBlock], mx = red[3 * kBlock];
     double cvv = 0.0;
@@ -489,28 +505,83 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, Run
 }
 
 // ---------------------------------------------------------------- Wiener --
-// LDS FP64 Stockham radix-2 FFT over P points; returns the buffer holding
-// the result (the other one is free).
+// In-register DFT of R = 2, 4 or 8 points (natural order in and out):
+// radix-2 decimation in time on the bit-reversed input.
+HD void dft_small(D2* v, int R, bool inverse) {
+    const double c = 0.70710678118654752440;  // sqrt(2)/2
+    const double sg = inverse ? 1.0 : -1.0;
+    D2 u[8];
+    if (R == 8) {
+        const int br[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+        for (int i = 0; i < 8; ++i) u[i] = v[br[i]];
+    } else if (R == 4) {
+        u[0] = v[0];
+        u[1] = v[2];
+        u[2] = v[1];
+        u[3] = v[3];
+    } else {
+        u[0] = v[0];
+        u[1] = v[1];
+    }
+    for (int i = 0; i < R; i += 2) {
+        const D2 a = u[i], b = u[i + 1];
+        u[i] = D2{a.x + b.x, a.y + b.y};
+        u[i + 1] = D2{a.x - b.x, a.y - b.y};
+    }
+    if (R >= 4) {
+        for (int g = 0; g < R; g += 4) {
+            const D2 a0 = u[g], a1 = u[g + 1], b0 = u[g + 2], b1 = u[g + 3];
+            // t = b1 * W4 with W4 = exp(sg * i * pi/2) = (0, sg)
+            const D2 tb1{-sg * b1.y, sg * b1.x};
+            u[g] = D2{a0.x + b0.x, a0.y + b0.y};
+            u[g + 2] = D2{a0.x - b0.x, a0.y - b0.y};
+            u[g + 1] = D2{a1.x + tb1.x, a1.y + tb1.y};
+            u[g + 3] = D2{a1.x - tb1.x, a1.y - tb1.y};
+        }
+    }
+    if (R == 8) {
+        // W8^j = exp(sg * i * pi * j / 4)
+        const D2 w[4] = {D2{1.0, 0.0}, D2{c, sg * c}, D2{0.0, sg}, D2{-c, sg * c}};
+        for (int j = 0; j < 4; ++j) {
+            const D2 t = cmul(u[j + 4], w[j]);
+            const D2 a = u[j];
+            u[j] = D2{a.x + t.x, a.y + t.y};
+            u[j + 4] = D2{a.x - t.x, a.y - t.y};
+        }
+    }
+    for (int i = 0; i < R; ++i) v[i] = u[i];
+}
+
+// LDS FP64 Stockham FFT over P = 2^k points with radix-8 stages (radix 4 or
+// 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P.  Returns the
+// buffer holding the result (the other one is free).
 HD D2* lds_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
-    const int half = P >> 1;
-    for (int Ns = 1; Ns < P; Ns <<= 1) {
+    for (int Ns = 1; Ns < P;) {
+        const int rem = P / Ns;
+        const int R = rem >= 8 ? 8 : rem;
+        const int nbf = P / R;
+        const int tstep = P / (Ns * R);
         BLOCK_FOR(t)
-        for (int j = t; j < half; j += kBlock) {
-            const D2 v0 = a[j];
-            const D2 v1 = a[j + half];
+        for (int j = t; j < nbf; j += kBlock) {
             const int k = j & (Ns - 1);
-            D2 w = tw[k * (half / Ns)];
-            if (inverse) w.y = -w.y;
-            const D2 tt = cmul(v1, w);
-            const int o = ((j - k) << 1) + k;
-            b[o] = D2{v0.x + tt.x, v0.y + tt.y};
-            b[o + Ns] = D2{v0.x - tt.x, v0.y - tt.y};
+            D2 v[8];
+            for (int r = 0; r < R; ++r) v[r] = a[j + r * nbf];
+            if (Ns > 1)
+                for (int r = 1; r < R; ++r) {
+                    D2 w = tw[r * k * tstep];
+                    if (inverse) w.y = -w.y;
+                    v[r] = cmul(v[r], w);
+                }
+            dft_small(v, R, inverse);
+            const int o = (j - k) * R + k;
+            for (int r = 0; r < R; ++r) b[o + r * Ns] = v[r];
         }
         BLOCK_END
         BLOCK_SYNC();
         D2* tmp = a;
         a = b;
         b = tmp;
+        Ns *= R;
     }
     return a;
 }
@@ -528,7 +599,7 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
     D2* z = cv.take<D2>(P);
     D2* y = cv.take<D2>(P);
     D2* ff = cv.take<D2>(P);
-    D2* tw = cv.take<D2>(half);
+    const D2* tw = rp.twiddle;
     const double bmin = st->bin_min, slope = st->slope;
     const double scaledFWHM = rp.fwhm / slope;
     const double expF = 4.0 * log(2.0) / (scaledFWHM * scaledFWHM);
@@ -539,11 +610,6 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
         const double Fj = (j == 0) ? scale : scale * exp((double)(-(int64_t)jj * jj) * expF);
         const double Vj = (j >= off && j < off + nb) ? (double)V.hist[j - off] * 0x1p-32 : 0.0;
         z[j] = D2{Vj, Fj};
-    }
-    for (int k = t; k < half; k += kBlock) {
-        double sn, cs;
-        sincospi_d(-2.0 * (double)k / (double)P, &sn, &cs);
-        tw[k] = D2{cs, sn};
     }
     BLOCK_END
     BLOCK_SYNC();
@@ -708,7 +774,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
         BLOCK_SYNC();
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
-            const int r = pr / Cz, c = pr - (pr / Cz) * Cz;
+            const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
             const int lo = S.zs[2 * c], hi = S.zs[2 * c + 1];
             const float* gr = S.g + (size_t)r * Z;
             float acc = 0.0f;
@@ -719,7 +785,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
         BLOCK_SYNC();
         BLOCK_FOR(t)
         for (int pr = t; pr < Cy * Cz; pr += kBlock) {
-            const int cy = pr / Cz, c = pr - (pr / Cz) * Cz;
+            const int cy = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - cy * Cz;
             int lo = S.ys[2 * cy], hi = S.ys[2 * cy + 1];
             lo = lo > y0 ? lo : y0;
             hi = hi < y0 + ny ? hi : y0 + ny;
@@ -872,7 +938,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
         const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
-            const int r = pr / Cz, c = pr - (pr / Cz) * Cz;
+            const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
             const AxisEntry& e = LD.ax[1][y0 + r];
             float b = 0.0f, br = 0.0f;
             for (int k = 0; k < 4; ++k) {
@@ -899,7 +965,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
             UNROLL for (int u = 0; u < 4; ++u) {
                 if (!ms[u]) continue;
                 const int v = v0 + u * kBlock + t;
-                const int r = v / Z, z = v - (v / Z) * Z;
+                const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
                 const float* wz = S.zw + 4 * z;
                 const float* br = S.B + r * Cz + S.zb[z];
                 const float* brr = S.Br + r * Cz + S.zb[z];
@@ -981,12 +1047,13 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
     }
     BLOCK_END
     BLOCK_SYNC();
-    for (int y0 = 0; y0 < Y; y0 += R) {
-        const int ny = (Y - y0 < R) ? Y - y0 : R;
+    const int yend = sl.y0 + sl.ny;
+    for (int y0 = sl.y0; y0 < yend; y0 += R) {
+        const int ny = (yend - y0 < R) ? yend - y0 : R;
         const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
-            const int r = pr / Cz, c = pr - (pr / Cz) * Cz;
+            const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
             const AxisEntry& e = V.fax[1][y0 + r];
             float b = 0.0f;
             for (int k = 0; k < 4; ++k) b += e.w[k] * S.A[(e.b + k) * Cz + c];
@@ -1004,7 +1071,7 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
             UNROLL for (int u = 0; u < 4; ++u) {
                 const int v = v0 + u * kBlock + t;
                 if (v >= ny * Z) continue;
-                const int r = v / Z, z = v - (v / Z) * Z;
+                const int r = (int)fdiv((uint32_t)v, V.mF2), z = v - r * Z;
                 const float* wz = S.zw + 4 * z;
                 const float* br = S.B + r * Cz + S.zb[z];
                 const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
@@ -1104,7 +1171,8 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
         tree_reduce4(red, 0, 0, 1, 2);
     BLOCK_FOR(t)
     if (t == 0) {
-        double* sp = V.spart + (size_t)sl.x * 4;
+        // min/max partials play the role of the prep partials (level 0 setup)
+        double* sp = (op == kFlatMinMax ? V.ppart : V.spart) + (size_t)sl.x * 4;
         sp[0] = red[0];
         sp[1] = red[kBlock];
         sp[2] = red[2 * kBlock];
@@ -1118,7 +1186,7 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
 size_t lds_prep() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_level() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_hist(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(uint64_t)); }
-size_t lds_wiener(int P) { return align16((3 * (size_t)P + P / 2) * sizeof(D2)); }
+size_t lds_wiener(int P) { return align16(3 * (size_t)P * sizeof(D2)); }
 size_t lds_lattice() { return align16(kBlock * sizeof(double)); }
 size_t lds_flat(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(double)); }
 

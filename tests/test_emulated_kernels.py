@@ -44,7 +44,8 @@ def test_n4_shrink_uneven(emu, shrink):
 
 
 def test_batch_equals_single(emu):
-    vols = [PC.small_phantom(d, seed=30 + k)[:2] for k, d in enumerate([(14, 12, 10), (9, 15, 11)])]
+    vols = [PC.small_phantom(d, seed=30 + k)[:2]
+            for k, d in enumerate([(14, 12, 10), (9, 15, 11), (12, 12, 12), (10, 13, 9)])]
     rb = emu.correct_batch(vols, levels=2, max_iter=[4, 4])
     for (im, m), r in zip(vols, rb):
         s = emu.correct(im, m, levels=2, max_iter=[4, 4])

@@ -124,10 +124,10 @@ def test_batch_equals_single(gpu):
     """One launch per stage over uneven volumes gives bitwise the same result
     as correcting each volume alone (same kernels, same reduction order)."""
     vols = []
-    for k, dims in enumerate([(30, 26, 22), (17, 33, 25), (41, 19, 28)]):
+    for k, dims in enumerate([(30, 26, 22), (17, 33, 25), (41, 19, 28), (24, 24, 24), (33, 29, 18), (20, 36, 31)]):
         im, m, _ = PC.small_phantom(dims, seed=20 + k)
         vols.append((im, m))
-    print("batch of 3 uneven volumes", flush=True)
+    print("batch of 6 uneven volumes (2 stream groups)", flush=True)
     rb = gpu.correct_batch(vols, levels=3, max_iter=[8, 8, 8])
     for (im, m), r in zip(vols, rb):
         s = gpu.correct(im, m, levels=3, max_iter=[8, 8, 8])
