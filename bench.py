@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (one MI355X); multi-GPU runs are not part "
@@ -104,11 +106,21 @@ def main():
         sys.exit(2)
 
     import n4amd
+    if not args.copy_calibration_mib:
+        # generate in worker processes before this process touches the GPU
+        t0 = time.time()
+        vols, cfg, shrink = make_workload(args.workload)
+        log(f"generated {len(vols)} volume(s) in {time.time() - t0:.1f}s")
     lib = n4amd.load()
     log(f"library {lib.version()} ({lib.path})")
-    t0 = time.time()
-    vols, cfg, shrink = make_workload(args.workload)
-    log(f"generated {len(vols)} volume(s) in {time.time() - t0:.1f}s")
+    if args.copy_calibration_mib:
+        nbytes = args.copy_calibration_mib << 20
+        a, b = lib.empty(nbytes), lib.empty(nbytes)
+        for _ in range(3):
+            lib._check(lib.L.n4_memcpy_dtod(lib.h, b.ptr, a.ptr, nbytes))
+        lib.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+        return
 
     p = n4amd.default_params(levels=cfg["levels"], max_iter=cfg["max_iterations"], threshold=0.001,
                              shrink=shrink)
@@ -184,9 +196,20 @@ def main():
         "iterations_per_level_mean": [round(float(np.mean([i[l] for i in its])), 2) for l in range(cfg["levels"])],
         "precision": "FP32 fit/evaluation, FP64 reductions and Wiener step",
     }
+    traffic, traffic_src = None, None
+    tpath = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        t = tj.get("per_launch", {}).get(dominant)
+        if t and t.get("fetch_bytes") is not None:
+            traffic = round(t["fetch_bytes"] + (t.get("write_bytes") or 0.0))
+            traffic_src = tj.get("source")
     result["roofline"] = {
         "bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-        "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+        "traffic_source": traffic_src,
+        "algorithmic_bytes_per_launch": round(dom_bytes / max(1, dom_n)),
         "avg_launch_us": round(dom_ms / max(1, dom_n) * 1e3, 3), "launches": int(dom_n),
     }
     result["kernel_breakdown_ms_per_step"] = {k: round(v[0] / max(1, args.warmup), 4) for k, v in breakdown.items()}

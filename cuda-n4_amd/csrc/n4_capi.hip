@@ -621,7 +621,7 @@ static n4_status poll_done(n4_handle* h, int g, int v0, int v1, int level, int i
 // Volumes are dealt to groups largest first (working voxels), each to the
 // group with the least work so far.
 static int group_count(int nvol) {
-    int g = nvol >= 16 ? 4 : (nvol >= 4 ? 2 : 1);
+    int g = nvol >= 4 ? 2 : 1;
     if (const char* e = getenv("N4_GROUPS")) {
         const int v = atoi(e);
         if (v >= 1) g = v;
@@ -1304,6 +1304,16 @@ n4_status n4_memcpy_dtoh(n4_handle* h, void* dst, const void* src, size_t bytes)
     if (!h || (bytes && (!dst || !src))) return fail(N4_ERR_INVALID_ARG, "NULL argument");
 #ifndef N4_EMULATE
     N4_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+#else
+    memcpy(dst, src, bytes);
+#endif
+    return N4_OK;
+}
+
+n4_status n4_memcpy_dtod(n4_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h || (bytes && (!dst || !src))) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
 #else
     memcpy(dst, src, bytes);
 #endif

@@ -298,6 +298,12 @@ HD void body_prep(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
 }
 
 // ------------------------------------------------------------ level setup --
+// Dyadic refinement weights (N4.py:134-136): even child {1/8, 6/8, 1/8},
+// odd child {0, 1/2, 1/2} over parents i-1, i, i+1.
+HD double refine_w(int odd, int a) {
+    return odd ? (a == 0 ? 0.0 : 0.5) : (a == 1 ? 0.75 : 0.125);
+}
+
 // One block per volume.
 //  1. level > 0: commits the previous level's last iteration (prev_iters
 //     enqueued by the host) unless the volume already stopped.
@@ -355,7 +361,6 @@ HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp,
     } else {
         const LevelDesc& PD = V.lev[level - 1];
         const double* src = V.lat[(level - 1) & 1];
-        const double bw[2][3] = {{1.0 / 8, 6.0 / 8, 1.0 / 8}, {0.0, 1.0 / 2, 1.0 / 2}};
         BLOCK_FOR(t)
         for (int l = t; l < LD.L; l += kBlock) {
             int c[3];
@@ -382,7 +387,7 @@ HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp,
                         int Z = par[2] - 1 + e;
                         const int nz = PD.C[2] - 3;
                         Z = Z < -1 ? -1 : (Z > nz + 1 ? nz + 1 : Z);
-                        const double wgt = bw[odd[0]][a] * bw[odd[1]][b] * bw[odd[2]][e];
+                        const double wgt = refine_w(odd[0], a) * refine_w(odd[1], b) * refine_w(odd[2], e);
                         acc += wgt * src[((size_t)(X + 1) * PD.C[1] + (Y + 1)) * PD.C[2] + (Z + 1)];
                     }
                 }
@@ -506,77 +511,86 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, Run
 
 // ---------------------------------------------------------------- Wiener --
 // In-register DFT of R = 2, 4 or 8 points (natural order in and out):
-// radix-2 decimation in time on the bit-reversed input.
-HD void dft_small(D2* v, int R, bool inverse) {
+// radix-2 decimation in time on the bit-reversed input.  R is a template
+// parameter so every index is a compile-time constant (registers only).
+template <int R>
+HD void dft_small(D2 (&v)[R], double sg) {
     const double c = 0.70710678118654752440;  // sqrt(2)/2
-    const double sg = inverse ? 1.0 : -1.0;
-    D2 u[8];
-    if (R == 8) {
-        const int br[8] = {0, 4, 2, 6, 1, 5, 3, 7};
-        for (int i = 0; i < 8; ++i) u[i] = v[br[i]];
-    } else if (R == 4) {
-        u[0] = v[0];
-        u[1] = v[2];
-        u[2] = v[1];
-        u[3] = v[3];
+    D2 u[R];
+    if constexpr (R == 8) {
+        u[0] = v[0]; u[1] = v[4]; u[2] = v[2]; u[3] = v[6];
+        u[4] = v[1]; u[5] = v[5]; u[6] = v[3]; u[7] = v[7];
+    } else if constexpr (R == 4) {
+        u[0] = v[0]; u[1] = v[2]; u[2] = v[1]; u[3] = v[3];
     } else {
-        u[0] = v[0];
-        u[1] = v[1];
+        u[0] = v[0]; u[1] = v[1];
     }
-    for (int i = 0; i < R; i += 2) {
+    UNROLL for (int i = 0; i < R; i += 2) {
         const D2 a = u[i], b = u[i + 1];
         u[i] = D2{a.x + b.x, a.y + b.y};
         u[i + 1] = D2{a.x - b.x, a.y - b.y};
     }
-    if (R >= 4) {
-        for (int g = 0; g < R; g += 4) {
+    if constexpr (R >= 4) {
+        UNROLL for (int g = 0; g < R; g += 4) {
             const D2 a0 = u[g], a1 = u[g + 1], b0 = u[g + 2], b1 = u[g + 3];
-            // t = b1 * W4 with W4 = exp(sg * i * pi/2) = (0, sg)
-            const D2 tb1{-sg * b1.y, sg * b1.x};
+            const D2 tb1{-sg * b1.y, sg * b1.x};  // b1 * exp(sg i pi/2)
             u[g] = D2{a0.x + b0.x, a0.y + b0.y};
             u[g + 2] = D2{a0.x - b0.x, a0.y - b0.y};
             u[g + 1] = D2{a1.x + tb1.x, a1.y + tb1.y};
             u[g + 3] = D2{a1.x - tb1.x, a1.y - tb1.y};
         }
     }
-    if (R == 8) {
-        // W8^j = exp(sg * i * pi * j / 4)
+    if constexpr (R == 8) {
+        // W8^j = exp(sg i pi j / 4)
         const D2 w[4] = {D2{1.0, 0.0}, D2{c, sg * c}, D2{0.0, sg}, D2{-c, sg * c}};
-        for (int j = 0; j < 4; ++j) {
+        UNROLL for (int j = 0; j < 4; ++j) {
             const D2 t = cmul(u[j + 4], w[j]);
             const D2 a = u[j];
             u[j] = D2{a.x + t.x, a.y + t.y};
             u[j + 4] = D2{a.x - t.x, a.y - t.y};
         }
     }
-    for (int i = 0; i < R; ++i) v[i] = u[i];
+    UNROLL for (int i = 0; i < R; ++i) v[i] = u[i];
 }
 
-// LDS FP64 Stockham FFT over P = 2^k points with radix-8 stages (radix 4 or
-// 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P.  Returns the
-// buffer holding the result (the other one is free).
+// One Stockham pass of radix R over P points (span Ns).
+template <int R>
+HD void fft_pass(const D2* a, D2* b, const D2* tw, int P, int Ns, bool inverse) {
+    const int nbf = P / R;
+    const int tstep = P / (Ns * R);
+    const double sg = inverse ? 1.0 : -1.0;
+    BLOCK_FOR(t)
+    for (int j = t; j < nbf; j += kBlock) {
+        const int k = j & (Ns - 1);
+        D2 v[R];
+        UNROLL for (int r = 0; r < R; ++r) v[r] = a[j + r * nbf];
+        if (Ns > 1) {
+            UNROLL for (int r = 1; r < R; ++r) {
+                D2 w = tw[r * k * tstep];
+                if (inverse) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_small<R>(v, sg);
+        const int o = (j - k) * R + k;
+        UNROLL for (int r = 0; r < R; ++r) b[o + r * Ns] = v[r];
+    }
+    BLOCK_END
+}
+
+// LDS FP64 Stockham FFT over P = 2^k points with radix-8 passes (radix 4
+// or 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P, in LDS.
+// Returns the buffer holding the result (the other one is free).
 HD D2* lds_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
     for (int Ns = 1; Ns < P;) {
         const int rem = P / Ns;
         const int R = rem >= 8 ? 8 : rem;
-        const int nbf = P / R;
-        const int tstep = P / (Ns * R);
-        BLOCK_FOR(t)
-        for (int j = t; j < nbf; j += kBlock) {
-            const int k = j & (Ns - 1);
-            D2 v[8];
-            for (int r = 0; r < R; ++r) v[r] = a[j + r * nbf];
-            if (Ns > 1)
-                for (int r = 1; r < R; ++r) {
-                    D2 w = tw[r * k * tstep];
-                    if (inverse) w.y = -w.y;
-                    v[r] = cmul(v[r], w);
-                }
-            dft_small(v, R, inverse);
-            const int o = (j - k) * R + k;
-            for (int r = 0; r < R; ++r) b[o + r * Ns] = v[r];
-        }
-        BLOCK_END
+        if (R == 8)
+            fft_pass<8>(a, b, tw, P, Ns, inverse);
+        else if (R == 4)
+            fft_pass<4>(a, b, tw, P, Ns, inverse);
+        else
+            fft_pass<2>(a, b, tw, P, Ns, inverse);
         BLOCK_SYNC();
         D2* tmp = a;
         a = b;
@@ -599,7 +613,7 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
     D2* z = cv.take<D2>(P);
     D2* y = cv.take<D2>(P);
     D2* ff = cv.take<D2>(P);
-    const D2* tw = rp.twiddle;
+    D2* tw = cv.take<D2>(P);
     const double bmin = st->bin_min, slope = st->slope;
     const double scaledFWHM = rp.fwhm / slope;
     const double expF = 4.0 * log(2.0) / (scaledFWHM * scaledFWHM);
@@ -610,6 +624,7 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
         const double Fj = (j == 0) ? scale : scale * exp((double)(-(int64_t)jj * jj) * expF);
         const double Vj = (j >= off && j < off + nb) ? (double)V.hist[j - off] * 0x1p-32 : 0.0;
         z[j] = D2{Vj, Fj};
+        tw[j] = rp.twiddle[j];
     }
     BLOCK_END
     BLOCK_SYNC();
@@ -1186,7 +1201,7 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
 size_t lds_prep() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_level() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_hist(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(uint64_t)); }
-size_t lds_wiener(int P) { return align16(3 * (size_t)P * sizeof(D2)); }
+size_t lds_wiener(int P) { return align16(4 * (size_t)P * sizeof(D2)); }
 size_t lds_lattice() { return align16(kBlock * sizeof(double)); }
 size_t lds_flat(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(double)); }
 

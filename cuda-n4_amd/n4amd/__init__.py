@@ -33,7 +33,8 @@ EXPORTS = [
     "n4_histogram", "n4_sharpen_image", "n4_fit_bspline3d", "n4_evaluate_bspline3d",
     "n4_convergence_measurement", "n4_upsample_lattice3d", "n4_reduce_sum", "n4_reduce_max",
     "n4_reduce_min", "n4_device_count", "n4_malloc", "n4_free", "n4_memcpy_htod",
-    "n4_memcpy_dtoh", "n4_memset", "n4_synchronize", "n4_profile_enable", "n4_profile_read",
+    "n4_memcpy_dtoh", "n4_memcpy_dtod", "n4_memset", "n4_synchronize", "n4_profile_enable",
+    "n4_profile_read",
 ]
 KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice"]
 
@@ -114,6 +115,7 @@ def _declare(L):
         "n4_free": ([vp, vp], C.c_int),
         "n4_memcpy_htod": ([vp, vp, vp, C.c_size_t], C.c_int),
         "n4_memcpy_dtoh": ([vp, vp, vp, C.c_size_t], C.c_int),
+        "n4_memcpy_dtod": ([vp, vp, vp, C.c_size_t], C.c_int),
         "n4_memset": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
         "n4_synchronize": ([vp], C.c_int),
         "n4_profile_enable": ([vp, u32], C.c_int),

@@ -203,6 +203,7 @@ n4_status n4_malloc(n4_handle *h, void **d_ptr, size_t bytes);
 n4_status n4_free(n4_handle *h, void *d_ptr);
 n4_status n4_memcpy_htod(n4_handle *h, void *d_dst, const void *h_src, size_t bytes);
 n4_status n4_memcpy_dtoh(n4_handle *h, void *h_dst, const void *d_src, size_t bytes);
+n4_status n4_memcpy_dtod(n4_handle *h, void *d_dst, const void *d_src, size_t bytes);
 n4_status n4_memset(n4_handle *h, void *d_dst, int value, size_t bytes);
 n4_status n4_synchronize(n4_handle *h);
 

@@ -1,0 +1,73 @@
+"""CLI (reference main.cpp flags) and NIfTI I/O.  CPU tests drive the
+CUDAN4_emu build (the same driver over the host emulation of the kernels);
+the GPU test drives the product CUDAN4 binary."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from n4amd import nifti
+import parity_cases as PC
+
+BUILD = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cuda-n4_amd", "_build")
+
+
+def run_cli(exe, args):
+    return subprocess.run([os.path.join(BUILD, exe)] + args, capture_output=True, text=True, timeout=300)
+
+
+def test_nifti_roundtrip(tmp_path):
+    a = np.random.default_rng(0).standard_normal((5, 6, 7)).astype(np.float32)
+    for name in ("a.nii", "a.nii.gz"):
+        p = str(tmp_path / name)
+        nifti.write(p, a)
+        assert np.array_equal(nifti.read(p), a)
+
+
+def test_cli_usage_and_errors(tmp_path):
+    r = run_cli("CUDAN4_emu", [])
+    assert r.returncode == 255 and "Usage" in r.stdout
+    r = run_cli("CUDAN4_emu", ["-q"])
+    assert r.returncode == 255 and "Unknown option" in r.stderr
+    r = run_cli("CUDAN4_emu", ["-i", str(tmp_path / "missing.nii"), "-o", str(tmp_path / "o.nii")])
+    assert r.returncode == 255 and "does not exist" in r.stderr
+    im, m, _ = PC.small_phantom((12, 10, 9))
+    nifti.write(str(tmp_path / "im.nii"), im)
+    nifti.write(str(tmp_path / "m.nii"), m[:, :, :8])
+    r = run_cli("CUDAN4_emu", ["-i", str(tmp_path / "im.nii"), "-m", str(tmp_path / "m.nii"), "-o",
+                               str(tmp_path / "o.nii")])
+    assert r.returncode == 255 and "Size mismatch" in r.stdout
+    r = run_cli("CUDAN4_emu", ["-i", str(tmp_path / "im.nii"), "-o", str(tmp_path / "o.nii"), "-n", "x"])
+    assert r.returncode == 255 and "Expected an integer" in r.stderr
+
+
+def _cli_matches_oracle(exe, tmp_path, dims, tol):
+    im, m, _ = PC.small_phantom(dims, seed=4)
+    pi, pm = str(tmp_path / "im.nii.gz"), str(tmp_path / "mask.nii.gz")
+    nifti.write(pi, im)
+    nifti.write(pm, m)
+    po, pb, pl, pr = (str(tmp_path / n) for n in ("norm.nii.gz", "bias.nii", "lat.nii", "rep.json"))
+    r = run_cli(exe, ["-i", pi, "-m", pm, "-o", po, "-b", pb, "-n", "8", "-t", "3", "-c", "4",
+                      "--lattice", pl, "--report", pr])
+    assert r.returncode == 0, r.stderr
+    ro = O.n4(im, m, levels=3, max_iter=8)
+    rep = json.load(open(pr))
+    assert rep["iterations"] == ro["iterations"]
+    mm = (m != 0) & (im > 10)
+    bias = nifti.read(pb)
+    assert np.abs(np.log(bias[mm]) - ro["log_bias"][mm]).max() <= tol
+    norm = nifti.read(po)
+    assert np.allclose(norm, ro["normalized"], rtol=1e-5, atol=1e-5)
+    assert np.abs(nifti.read(pl) - ro["lattice"]).max() <= 10 * tol
+
+
+def test_cli_emulated_matches_oracle(tmp_path):
+    _cli_matches_oracle("CUDAN4_emu", tmp_path, (30, 26, 22), PC.LOGBIAS_TOL)
+
+
+@pytest.mark.gpu
+def test_cli_gpu_matches_oracle(tmp_path):
+    _cli_matches_oracle("CUDAN4", tmp_path, (48, 44, 40), PC.LOGBIAS_TOL)
