@@ -7,8 +7,10 @@ the multi-level fitting loop (histogram sharpening with the Wiener step,
 B-spline fit, field update, convergence) and the full-resolution field +
 normalisation.  Default workload: 64 mixed-size head-class phantoms, each
 axis drawn from U{128..288}, shrink 4, 4 levels x <= 50 iterations, one
-launch per stage across all volumes.  The plain single-volume FP32 path is
-timed on the same volumes, and the FP64 CPU oracle on a bounded sample.
+launch per stage across all volumes.  The batched path is timed with FP32,
+BF16-MFMA and split-BF16-MFMA B-spline contractions (headline: --precision),
+each checked against the FP64 CPU oracle on a bounded sample of the volumes;
+the plain single-volume FP32 path is timed on the same volumes.
 
 Prints ONE JSON line (rank 0).  Progress lines go to stderr.
 """
@@ -60,6 +62,12 @@ def make_workload(name):
                            "(64x64x44 working grid), 4 levels x <=50 iterations, threshold 0.001",
                "volumes": 1, "shrink": 4, "levels": 4, "max_iterations": 50}
         return [(im, mask)], cfg, 4
+    if name == "c2":
+        im, mask, _ = P.c2_phantom(2)
+        cfg = {"workload": "512x512x320 7T-like head phantom, bias lattice 6^3 N(0,0.4), shrink 2 "
+                           "(256x256x160 working grid), 4 levels x <=50 iterations, threshold 0.001",
+               "volumes": 1, "shrink": 2, "levels": 4, "max_iterations": 50}
+        return [(im, mask)], cfg, 2
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -88,12 +96,34 @@ def algorithmic_bytes(kernel, vols_meta, reports, levels):
     return total
 
 
+def oracle_sample(vols, cfg, shrink, cpu_seconds, nthreads):
+    """FP64 oracle on the first volumes of the workload until cpu_seconds of
+    wall time: the CPU baseline and the error reference."""
+    import oracle as O
+    out, t_cpu = [], 0.0
+    for i, (im, mask) in enumerate(vols):
+        t1 = time.perf_counter()
+        ro = O.n4(im, mask, levels=cfg["levels"], max_iter=[cfg["max_iterations"]] * cfg["levels"],
+                  conv_threshold=0.001, shrink=shrink, nthreads=nthreads)
+        dt = time.perf_counter() - t1
+        t_cpu += dt
+        out.append({"i": i, "log_bias": ro["log_bias"], "iterations": ro["iterations"]})
+        log(f"oracle volume {i}: {dt:.2f}s, iterations {ro['iterations']}")
+        if t_cpu >= cpu_seconds:
+            break
+    return out, t_cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch64", choices=["batch64", "c1"])
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "c1", "c2"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "split_bf16"],
+                    help="precision of the headline batched figure")
+    ap.add_argument("--precisions", default="fp32,bf16,split_bf16",
+                    help="batched precisions timed (comma list; must include --precision)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -104,6 +134,9 @@ def main():
         print("bench.py: this build runs on exactly one GPU (one MI355X); multi-GPU runs are not part "
               "of it. Re-run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    precs = [q for q in args.precisions.split(",") if q]
+    if args.precision not in precs:
+        precs.append(args.precision)
 
     import n4amd
     if not args.copy_calibration_mib:
@@ -111,6 +144,11 @@ def main():
         t0 = time.time()
         vols, cfg, shrink = make_workload(args.workload)
         log(f"generated {len(vols)} volume(s) in {time.time() - t0:.1f}s")
+        # one core for the batch (N4.py's execution model); all host cores (at
+        # most 16) for the single large volume, which one core cannot finish
+        cpu_threads = 1 if len(vols) > 1 else min(16, os.cpu_count() or 1)
+        sample, t_cpu = (([], 0.0) if args.no_cpu_baseline
+                         else oracle_sample(vols, cfg, shrink, args.cpu_seconds, cpu_threads))
     lib = n4amd.load()
     log(f"library {lib.version()} ({lib.path})")
     if args.copy_calibration_mib:
@@ -122,9 +160,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return
 
-    p = n4amd.default_params(levels=cfg["levels"], max_iter=cfg["max_iterations"], threshold=0.001,
-                             shrink=shrink)
-    nc = n4amd.final_ncpt(p)
+    base_kw = dict(levels=cfg["levels"], max_iter=cfg["max_iterations"], threshold=0.001, shrink=shrink)
+    nc = n4amd.final_ncpt(n4amd.default_params(**base_kw))
     L = nc[0] * nc[1] * nc[2]
     # device-resident inputs and outputs
     dev, meta = [], []
@@ -147,37 +184,67 @@ def main():
     nvol = len(vols)
     reps = (n4amd.Report * nvol)()
 
-    def batch_step():
+    def batch_step(p):
         lib._check(lib.L.n4_correct_batch(lib.h, arr, nvol, C.byref(p), reps, None))
 
-    # warmup with every kernel timed: picks the dominant kernel
-    lib.profile_enable(n4amd.KERNELS)
-    for _ in range(max(1, args.warmup)):
-        batch_step()
-    breakdown = {k: lib.profile_read(k) for k in n4amd.KERNELS}
-    dominant = max(breakdown, key=lambda k: breakdown[k][0])
-    log("warmup per-kernel device time (ms, launches): "
-        + ", ".join(f"{k}={v[0] / max(1, args.warmup):.3f}/{v[1] // max(1, args.warmup)}" for k, v in breakdown.items()))
-    log(f"dominant kernel: {dominant}")
+    def sample_errors(its):
+        errs, matched = [], 0
+        for smp in sample:
+            i = smp["i"]
+            im, mask = vols[i]
+            lb = dev[i][3].download(np.float32, im.shape).astype(np.float64)
+            mm = (mask != 0) & (im > 10)
+            e = np.abs(lb - smp["log_bias"])[mm]
+            errs.append((float(e.max()), float(np.sqrt(np.mean(e ** 2)))))
+            matched += int(its[i] == smp["iterations"])
+        if not errs:
+            return None
+        return {"sample_volumes": len(errs), "max": max(e[0] for e in errs), "rms_max": max(e[1] for e in errs),
+                "iteration_counts_matched": f"{matched}/{len(errs)}"}
 
-    # timed region: only the dominant kernel carries events
-    lib.profile_enable([dominant])
-    lib.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch_step()
-    lib.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = 1e3 * dt / args.steps
-    dom_ms, dom_n = lib.profile_read(dominant)
-    its = [[int(r.iterations[l]) for l in range(cfg["levels"])] for r in reps]
-    vox_iter = sum(m["n_work"] * sum(i) for m, i in zip(meta, its))
-    value = nvol * args.steps / dt
-    dom_bytes = algorithmic_bytes(dominant, meta, reps, cfg["levels"]) * args.steps
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    log(f"batched: {ms_per_step:.3f} ms/step, {value:.1f} volumes/s; dominant {dominant}: "
-        f"{dom_ms / max(1, dom_n) * 1e3:.2f} us avg over {dom_n} launches, {achieved:.1f} GB/s")
+    runs = {}
+    for prec in precs:
+        p = n4amd.default_params(precision=prec, **base_kw)
+        # warmup with every kernel timed: picks the dominant kernel
+        lib.profile_enable(n4amd.KERNELS)
+        for _ in range(max(1, args.warmup)):
+            batch_step(p)
+        breakdown = {k: lib.profile_read(k) for k in n4amd.KERNELS}
+        dominant = max(breakdown, key=lambda k: breakdown[k][0])
+        log(f"[{prec}] warmup per-kernel device time (ms, launches): "
+            + ", ".join(f"{k}={v[0] / max(1, args.warmup):.3f}/{v[1] // max(1, args.warmup)}"
+                        for k, v in breakdown.items()))
+        # timed region: only the dominant kernel carries events
+        lib.profile_enable([dominant])
+        lib.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch_step(p)
+        lib.synchronize()
+        dt = time.perf_counter() - t0
+        dom_ms, dom_n = lib.profile_read(dominant)
+        its = [[int(r.iterations[l]) for l in range(cfg["levels"])] for r in reps]
+        vox_iter = sum(m["n_work"] * sum(i) for m, i in zip(meta, its))
+        dom_bytes = algorithmic_bytes(dominant, meta, reps, cfg["levels"]) * args.steps
+        achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+        R = {"volumes_per_s": nvol * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+             "mvoxel_iter_per_s": vox_iter * args.steps / dt / 1e6,
+             "iterations_per_level_mean": [round(float(np.mean([i[l] for i in its])), 2)
+                                           for l in range(cfg["levels"])],
+             "dominant": dominant, "dom_ms": dom_ms, "dom_n": dom_n, "dom_bytes": dom_bytes, "achieved": achieved,
+             "breakdown": {k: round(v[0] / max(1, args.warmup), 4) for k, v in breakdown.items()},
+             "err": sample_errors(its)}
+        runs[prec] = R
+        log(f"[{prec}] batched: {R['ms_per_step']:.3f} ms/step, {R['volumes_per_s']:.1f} volumes/s; dominant "
+            f"{dominant}: {dom_ms / max(1, dom_n) * 1e3:.2f} us avg over {dom_n} launches, {achieved:.1f} GB/s; "
+            f"err {R['err']}")
 
+    H = runs[args.precision]
+    value = H["volumes_per_s"]
+    prec_text = {"fp32": "FP32 B-spline fit/evaluation/reconstruction",
+                 "bf16": "BF16-input MFMA B-spline fit/evaluation/reconstruction with FP32 accumulation",
+                 "split_bf16": "split-BF16 (hi+lo) MFMA B-spline fit/evaluation/reconstruction with FP32 "
+                               "accumulation"}
     result = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -185,34 +252,42 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(H["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": {"fp32": "fp32", "bf16": "bf16", "split_bf16": "bf16x2"}[args.precision],
         "data": "synthetic phantoms (numpy default_rng seeds recorded in the generator), inputs resident in HBM",
-        "config": cfg,
-        "mvoxel_iter_per_s": round(vox_iter * args.steps / dt / 1e6, 2),
-        "iterations_per_level_mean": [round(float(np.mean([i[l] for i in its])), 2) for l in range(cfg["levels"])],
-        "precision": "FP32 fit/evaluation, FP64 reductions and Wiener step",
+        "config": dict(cfg, precision=args.precision),
+        "mvoxel_iter_per_s": round(H["mvoxel_iter_per_s"], 2),
+        "iterations_per_level_mean": H["iterations_per_level_mean"],
+        "precision": prec_text[args.precision] + "; FP64 histogram range, Wiener step, lattice update and CV sums",
     }
+    dominant, dom_n = H["dominant"], H["dom_n"]
     traffic, traffic_src = None, None
     tpath = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        t = tj.get("per_launch", {}).get(dominant)
+        t = tj.get("per_launch", {}).get(args.precision, {}).get(dominant)
         if t and t.get("fetch_bytes") is not None:
             traffic = round(t["fetch_bytes"] + (t.get("write_bytes") or 0.0))
             traffic_src = tj.get("source")
     result["roofline"] = {
-        "bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-        "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+        "bound": "hbm", "kernel": dominant, "achieved": round(H["achieved"], 2), "peak": PEAK_HBM_GBS,
+        "unit": "GB/s", "frac": round(H["achieved"] / PEAK_HBM_GBS, 5), "traffic": traffic,
         "traffic_source": traffic_src,
-        "algorithmic_bytes_per_launch": round(dom_bytes / max(1, dom_n)),
-        "avg_launch_us": round(dom_ms / max(1, dom_n) * 1e3, 3), "launches": int(dom_n),
+        "algorithmic_bytes_per_launch": round(H["dom_bytes"] / max(1, dom_n)),
+        "avg_launch_us": round(H["dom_ms"] / max(1, dom_n) * 1e3, 3), "launches": int(dom_n),
     }
-    result["kernel_breakdown_ms_per_step"] = {k: round(v[0] / max(1, args.warmup), 4) for k, v in breakdown.items()}
+    result["kernel_breakdown_ms_per_step"] = H["breakdown"]
+    result["batched"] = {q: {"volumes_per_s": round(R["volumes_per_s"], 3), "ms_per_step": round(R["ms_per_step"], 4),
+                             "mvoxel_iter_per_s": round(R["mvoxel_iter_per_s"], 2),
+                             "logbias_err_vs_fp64": R["err"], "kernel_ms_per_step": R["breakdown"]}
+                         for q, R in runs.items()}
+    if H["err"] is not None:
+        result["logbias_err_vs_fp64"] = H["err"]
+    p_head = n4amd.default_params(precision=args.precision, **base_kw)
 
     # host-buffer rate: pageable host image + mask up, normalised + log bias down
     lib.profile_enable([])
@@ -221,7 +296,7 @@ def main():
     for (im, mask), d in zip(vols, dev):
         d[0].upload(im)
         d[1].upload(mask.astype(np.float32))
-    batch_step()
+    batch_step(p_head)
     host_out = [(np.empty_like(im), np.empty_like(im)) for im, _ in vols]
     for (a, b), d in zip(host_out, dev):
         lib._check(lib.L.n4_memcpy_dtoh(lib.h, a.ctypes.data, d[2].ptr, a.nbytes))
@@ -236,15 +311,15 @@ def main():
 
     # plain single-volume FP32 path over the same volumes
     if not args.no_plain:
-        lib.profile_enable([])
+        p32 = n4amd.default_params(precision="fp32", **base_kw)
         rep1 = n4amd.Report()
 
         def plain_pass():
             for i in range(nvol):
                 v = arr[i]
                 lib._check(lib.L.n4_correct(lib.h, v.d_im, v.d_mask, v.mask_kind, v.dims[0], v.dims[1], v.dims[2],
-                                            C.byref(p), v.d_im_normalized, v.d_lattice, None, v.d_log_bias_field,
-                                            C.byref(rep1), None))
+                                            C.byref(p32), v.d_im_normalized, v.d_lattice, None,
+                                            v.d_log_bias_field, C.byref(rep1), None))
         plain_pass()
         lib.synchronize()
         t0 = time.perf_counter()
@@ -254,37 +329,12 @@ def main():
         result["plain_fp32"] = {"volumes_per_s": round(nvol / pdt, 3), "ms_per_volume": round(1e3 * pdt / nvol, 4),
                                 "batched_over_plain": round(value / (nvol / pdt), 3)}
         log(f"plain single-volume path: {nvol / pdt:.1f} volumes/s")
-        batch_step()  # restore batched outputs for the error check
-        lib.synchronize()
 
-    # FP64 oracle on a bounded sample: CPU baseline and log-field error
-    if not args.no_cpu_baseline:
-        import oracle as O
-        done, errs, t_cpu, matched = 0, [], 0.0, 0
-        for i in range(nvol):
-            im, mask = vols[i]
-            t1 = time.perf_counter()
-            ro = O.n4(im, mask, levels=cfg["levels"], max_iter=[cfg["max_iterations"]] * cfg["levels"],
-                      conv_threshold=0.001, shrink=shrink, nthreads=1)
-            t_cpu += time.perf_counter() - t1
-            done += 1
-            lb = dev[i][3].download(np.float32, im.shape).astype(np.float64)
-            mm = (mask != 0) & (im > 10)
-            e = np.abs(lb - ro["log_bias"])[mm]
-            errs.append((float(e.max()), float(np.sqrt(np.mean(e ** 2)))))
-            matched += int(its[i] == ro["iterations"])
-            log(f"oracle volume {i}: {time.perf_counter() - t1:.2f}s, iterations {ro['iterations']} "
-                f"(gpu {its[i]}), max err {e.max():.3g}")
-            if t_cpu >= args.cpu_seconds:
-                break
+    if sample:
         result["cpu_baseline"] = {
-            "value": round(done / t_cpu, 4), "unit": "volumes/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} volume(s) of the workload, FP64 C restatement of N4.py (oracle/), "
-                      f"single thread, same levels/iterations/threshold/shrink",
-        }
-        result["logbias_err_vs_fp64"] = {
-            "sample_volumes": done, "max": max(e[0] for e in errs), "rms_max": max(e[1] for e in errs),
-            "iteration_counts_matched": f"{matched}/{done}",
+            "value": round(len(sample) / t_cpu, 4), "unit": "volumes/s", "cores": cpu_threads, "kind": "port",
+            "sample": f"first {len(sample)} volume(s) of the workload, FP64 C restatement of N4.py (oracle/), "
+                      f"{cpu_threads} thread(s), same levels/iterations/threshold/shrink",
         }
     print(json.dumps(result), flush=True)
 

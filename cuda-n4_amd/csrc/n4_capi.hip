@@ -80,7 +80,6 @@ static void dev_free(void* p) { free(p); }
 #endif
 
 // ---------------------------------------------------------------- handle --
-static const int kPipelineSlots = 4;
 static const int kMaxGroups = 8;
 
 struct n4_handle {
@@ -88,12 +87,11 @@ struct n4_handle {
     void* arena = nullptr;
     size_t arena_bytes = 0;
     std::vector<unsigned char> staging;  // host image of the upload region
-    int32_t* pinned_flags = nullptr;     // flag_cap level-tagged done flags (pinned host)
-    int32_t* dev_flags = nullptr;        // the same memory as seen by the device
+    uint64_t* pinned_flags = nullptr;    // flag_cap progress words (pinned host)
+    uint64_t* dev_flags = nullptr;       // the same memory as seen by the device
     int flag_cap = 0;
 #ifndef N4_EMULATE
-    hipEvent_t flag_ev[kMaxGroups][kPipelineSlots];
-    bool flag_ev_init = false;
+    bool events_init = false;
     hipStream_t gstream[kMaxGroups] = {};  // streams of batch groups 1..
     hipEvent_t fork_ev = nullptr, join_ev[kMaxGroups] = {};
     std::vector<hipEvent_t> prof_ev;       // event pool for kernel timing
@@ -104,12 +102,12 @@ struct n4_handle {
     uint64_t prof_n[N4_K_COUNT] = {0};
 };
 
-// Early exit: the convergence tail writes each volume's done flag, tagged
-// with level+1, straight into pinned host memory; the host records an event
-// per iteration and, kPipeline iterations behind the GPU, stops enqueueing
-// iterations once every volume is done at the current level.
+// Early exit: each histogram pass writes the volume's progress word (level,
+// passes run, done) straight into pinned host memory.  The host keeps at
+// most kPipeline iterations enqueued ahead of the slowest unfinished volume
+// and stops enqueueing once every volume is done at the current level; the
+// iteration stream carries no events.
 static const int kPipeline = 3;
-static const int kFlagSlots = kPipelineSlots;
 
 static n4_status ensure_arena(n4_handle* h, size_t bytes) {
     if (bytes <= h->arena_bytes) return N4_OK;
@@ -191,6 +189,7 @@ static RunParams make_run_params(const n4_params* p) {
     if ((double)f > p->low_value) f = nextafterf(f, -INFINITY);
     rp.low_f = f;
     rp.fixed_iters = (p->flags & N4_FLAG_FIXED_ITERATIONS) ? 1 : 0;
+    rp.prec = (int)p->precision;
     return rp;
 }
 
@@ -231,6 +230,7 @@ struct Plan {
     std::vector<int> recon_tiles;  // recon tiles per volume
     size_t lds_fit_max[kMaxLevels] = {0}, lds_eval_max[kMaxLevels] = {0};
     int cx_max[kMaxLevels] = {0};
+    int mt[kMaxLevels] = {0}, nt[kMaxLevels] = {0};  // MFMA tiles along y, z per level
     size_t lds_hist = 0, lds_wiener = 0, lds_recon = 0;
 };
 
@@ -282,6 +282,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         I.prep_rows = rows_for(I.W[1], I.W[2], kPrepTileVox);
         I.prep_tiles_x = (I.W[1] + I.prep_rows - 1) / I.prep_rows;
         I.recon_rows = rows_for(in[v].F[1], in[v].F[2], kReconTileVox);
+        // MFMA recon: four waves of 16 rows each per workgroup
+        if (P->rp.prec) I.recon_rows = std::min(in[v].F[1], (I.recon_rows + 63) / 64 * 64);
         I.recon_tiles_x = (in[v].F[1] + I.recon_rows - 1) / I.recon_rows;
         nws += I.W[0];
         nps += I.W[0] * I.prep_tiles_x;
@@ -315,7 +317,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
 
     // per-volume working buffers
     struct Offs {
-        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, hist, spart, etab, ppart;
+        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, numfx, hist, spart, etab, ppart;
     };
     std::vector<Offs> off(nv);
     const int nb = P->rp.nbins;
@@ -332,6 +334,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.latf = lay.take(sizeof(float) * I.Lmax);
         o.lresf = lay.take(sizeof(float) * I.Lmax);
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
+        o.numfx = lay.take(sizeof(uint64_t) * 2 * I.Lmax);
         o.hist = lay.take(sizeof(uint64_t) * nb);
         o.spart = lay.take(sizeof(double) * 4 * I.W[0]);
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
@@ -386,6 +389,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.latf = (float*)(base + o.latf);
         D.lresf = (float*)(base + o.lresf);
         D.part = (float*)(base + o.part);
+        D.numfx[0] = (uint64_t*)(base + o.numfx);
+        D.numfx[1] = D.numfx[0] + I.Lmax;
         D.hist = (uint64_t*)(base + o.hist);
         D.spart = (double*)(base + o.spart);
         D.ppart = (double*)(base + o.ppart);
@@ -395,7 +400,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
         D.st = (VolState*)(base + P->off_state) + v;
-        D.done_flag = (h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
+        D.progress = (h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
         D.nlevels = L;
         D.part_stride = (int32_t)I.part_stride;
         for (int l = 0; l < L; ++l) {
@@ -431,12 +436,15 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             for (int x = 0; x < in[v].F[0]; ++x)
                 for (int y0 = 0; y0 < in[v].F[1]; y0 += I.recon_rows)
                     fs[ifs++] = Slab{v, x, y0, std::min(I.recon_rows, in[v].F[1] - y0)};
+        const int prec = P->rp.prec;
         for (int l = 0; l < L; ++l) {
-            P->lds_fit_max[l] = std::max(P->lds_fit_max[l], lds_fit(nb, D, l));
-            P->lds_eval_max[l] = std::max(P->lds_eval_max[l], lds_eval(D, l));
+            P->lds_fit_max[l] = std::max(P->lds_fit_max[l], lds_fit(nb, D, l, prec));
+            P->lds_eval_max[l] = std::max(P->lds_eval_max[l], lds_eval(D, l, prec));
             P->cx_max[l] = std::max(P->cx_max[l], D.lev[l].C[0]);
+            P->mt[l] = mfma_tiles(D.lev[l].C[1]);
+            P->nt[l] = mfma_tiles(D.lev[l].C[2]);
         }
-        if (need_tables_full) P->lds_recon = std::max(P->lds_recon, lds_recon(D));
+        if (need_tables_full) P->lds_recon = std::max(P->lds_recon, lds_recon(D, I.recon_rows, prec));
     }
     P->lds_hist = lds_hist(nb);
     P->lds_wiener = lds_wiener(P->rp.P);
@@ -491,9 +499,20 @@ static n4_status check_params(const n4_params* p) {
     if (!(p->wiener_filter_noise >= 0.0)) return fail(N4_ERR_INVALID_ARG, "wiener_filter_noise must be >= 0");
     if (!(p->fit_eps >= 0.0)) return fail(N4_ERR_INVALID_ARG, "fit_eps must be >= 0");
     if (p->shrink_factor < 1 || p->shrink_factor > 64) return fail(N4_ERR_INVALID_ARG, "shrink_factor must be in [1, 64]");
+    if (p->precision != N4_PREC_FP32 && p->precision != N4_PREC_BF16 && p->precision != N4_PREC_SPLIT_BF16)
+        return fail(N4_ERR_INVALID_ARG, "precision %u is not an n4_precision value", p->precision);
+#ifdef N4_EMULATE
     if (p->precision != N4_PREC_FP32)
-        return fail(N4_ERR_UNSUPPORTED, "precision %u: only N4_PREC_FP32 is implemented in this build",
+        return fail(N4_ERR_UNSUPPORTED, "precision %u: the host emulation build runs the FP32 kernels only",
                     p->precision);
+#endif
+    if (p->precision != N4_PREC_FP32)
+        for (int d = 1; d < 3; ++d) {
+            const uint64_t cf = ((uint64_t)(p->ncpt[d] - 3) << (p->number_of_fitting_levels - 1)) + 3;
+            if (cf > (uint64_t)(16 * kMfmaMaxTiles))
+                return fail(N4_ERR_UNSUPPORTED, "MFMA precision modes support up to %d control points along y and z "
+                            "(final level has %llu along axis %d)", 16 * kMfmaMaxTiles, (unsigned long long)cf, d);
+        }
     // final lattice size must stay sane
     uint64_t Lf = 1;
     for (int d = 0; d < 3; ++d) {
@@ -572,44 +591,55 @@ static n4_status ensure_flags(n4_handle* h, int nvol) {
     N4_CHECK_HIP(hipDeviceSynchronize());
     if (h->pinned_flags) (void)hipHostFree(h->pinned_flags);
     h->pinned_flags = nullptr;
-    N4_CHECK_HIP(hipHostMalloc((void**)&h->pinned_flags, sizeof(int32_t) * nvol, hipHostMallocDefault));
+    N4_CHECK_HIP(hipHostMalloc((void**)&h->pinned_flags, sizeof(uint64_t) * nvol, hipHostMallocDefault));
     N4_CHECK_HIP(hipHostGetDevicePointer((void**)&h->dev_flags, h->pinned_flags, 0));
-    if (!h->flag_ev_init) {
-        for (int g = 0; g < kMaxGroups; ++g)
-            for (int k = 0; k < kFlagSlots; ++k)
-                N4_CHECK_HIP(hipEventCreateWithFlags(&h->flag_ev[g][k], hipEventDisableTiming));
+    if (!h->events_init) {
         N4_CHECK_HIP(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
         for (int g = 0; g < kMaxGroups; ++g) N4_CHECK_HIP(hipEventCreateWithFlags(&h->join_ev[g], hipEventDisableTiming));
-        h->flag_ev_init = true;
+        h->events_init = true;
     }
 #else
     free(h->pinned_flags);
-    h->pinned_flags = (int32_t*)calloc((size_t)nvol, sizeof(int32_t));
+    h->pinned_flags = (uint64_t*)calloc((size_t)nvol, sizeof(uint64_t));
     h->dev_flags = h->pinned_flags;
 #endif
     h->flag_cap = nvol;
     return N4_OK;
 }
 
-// Records the end of iteration `it` of group g and inspects the flags of
-// volumes [v0, v1) once iteration it - kPipeline has completed.  Flags only
-// move from 0 to level+1 within a level, so reading a newer state than that
-// iteration's is harmless.
-static n4_status poll_done(n4_handle* h, int g, int v0, int v1, int level, int it, void* stream, bool* all_done) {
-    *all_done = false;
+// Non-blocking check of a group (volumes [v0, v1)) that has iterations
+// [0, it) of `level` enqueued on `stream`: *ready once every volume has
+// finished iteration it - kPipeline - 1 (its histogram pass it - kPipeline + 1
+// has run) or is done; *all_done once every volume is done.  Progress only
+// moves forward within a level, so a newer state than the one asked for is
+// harmless.  With `query`, a drained stream makes the words final (this
+// also ends the wait if a stream stopped early).
+static n4_status check_progress(n4_handle* h, int v0, int v1, int level, int it, void* stream, bool query,
+                                bool* ready, bool* all_done) {
+    const int need = it - kPipeline + 1;
+    const volatile uint64_t* f = h->pinned_flags;
+    bool rdy = true, all = true;
+    for (int v = v0; v < v1; ++v) {
+        const uint64_t w = f[v];
+        const bool cur = progress_level(w) == level;
+        const bool done = cur && progress_done(w);
+        all = all && done;
+        rdy = rdy && (done || need <= 0 || (cur && progress_passes(w) >= need));
+    }
 #ifndef N4_EMULATE
-    N4_CHECK_HIP(hipEventRecord(h->flag_ev[g][it % kFlagSlots], (hipStream_t)stream));
-    const int old = it - kPipeline;
-    if (old < 0) return N4_OK;
-    N4_CHECK_HIP(hipEventSynchronize(h->flag_ev[g][old % kFlagSlots]));
+    if (!rdy && query) {
+        const hipError_t q = hipStreamQuery((hipStream_t)stream);
+        if (q == hipSuccess)
+            rdy = true;
+        else if (q != hipErrorNotReady)
+            return fail(N4_ERR_HIP, "HIP error %d (%s) while polling progress", (int)q, hipGetErrorString(q));
+    }
 #else
-    (void)g;
     (void)stream;
-    (void)it;
+    (void)query;
+    rdy = true;  // host emulation: every enqueued kernel has run
 #endif
-    const volatile int32_t* f = h->pinned_flags;
-    bool all = true;
-    for (int v = v0; v < v1 && all; ++v) all = f[v] == level + 1;
+    *ready = rdy;
     *all_done = all;
     return N4_OK;
 }
@@ -746,49 +776,53 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
         for (int g = 1; g < G; ++g) N4_CHECK_HIP(hipStreamWaitEvent(h->gstream[g], h->fork_ev, 0));
     }
 #endif
-    // round-robin over groups, one iteration (or level transition) at a time
+    // round-robin over groups, one iteration (or level transition) at a time;
+    // a group whose pipeline is full is skipped until its progress allows
+    // more, and the host spins only when no group can advance
     int remaining = G;
+    uint64_t idle_rounds = 0;
     while (remaining > 0) {
+        bool advanced = false;
         for (int g = 0; g < G; ++g) {
             GroupRun& R = gr[g];
             if (R.finished) continue;
             void* gs = R.stream;
+            bool level_end = R.it >= P.max_iter[R.level];
+            if (R.started && R.it > 0 && !level_end) {
+                bool ready = false, all_done = false;
+                s = check_progress(h, R.v0, R.v1, R.level, R.it, gs, (idle_rounds & 63) == 63, &ready, &all_done);
+                if (s != N4_OK) return s;
+                if (!ready) continue;
+                level_end = all_done;
+            }
             const int nvg = R.v1 - R.v0, nws = R.ws1 - R.ws0;
             const VolDesc* gv = dd + R.v0;
             const Slab* gsl = wsl + R.ws0;
             const int l = R.level;
             if (!R.started) {
                 GLAUNCH(N4_K_LEVEL, launch_level(gv, nvg, l, R.prev_iters, rp, gs));
-                GLAUNCH(N4_K_FIT_DEN, launch_fit(dd, gsl, nws, l, kFitDen, rp, P.lds_fit_max[l], gs));
+                GLAUNCH(N4_K_FIT_DEN, launch_fit(dd, gsl, nws, l, 0, kFitDen, rp, P.lds_fit_max[l], P.mt[l], P.nt[l], gs));
                 GLAUNCH(N4_K_LATTICE, launch_lattice(gv, nvg, P.cx_max[l], l, kFitDen, rp, gs));
                 R.started = true;
             }
-            bool level_end = (P.max_iter[l] <= 0);
+            advanced = true;
             if (!level_end) {
                 const int it = R.it;
                 GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, l, it, rp, nullptr, P.lds_hist, gs));
                 GLAUNCH(N4_K_WIENER, launch_wiener(gv, nvg, rp, P.lds_wiener, gs));
-                GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, l, kFitNum, rp, P.lds_fit_max[l], gs));
-                GLAUNCH(N4_K_LATTICE, launch_lattice(gv, nvg, P.cx_max[l], l, kFitNum, rp, gs));
-                GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, l, rp, P.lds_eval_max[l], gs));
+                GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, l, it, kFitNum, rp, P.lds_fit_max[l], P.mt[l], P.nt[l], gs));
+                GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, l, it, rp, P.lds_eval_max[l], P.mt[l], P.nt[l], gs));
                 R.it = it + 1;
-                if (R.it >= P.max_iter[l]) {
-                    level_end = true;
-                } else {
-                    bool all_done = false;
-                    s = poll_done(h, g, R.v0, R.v1, l, it, gs, &all_done);
-                    if (s != N4_OK) return s;
-                    level_end = all_done;
-                }
+                continue;
             }
-            if (!level_end) continue;
             R.prev_iters = R.it;
             R.it = 0;
             R.started = false;
             R.level = l + 1;
             if (R.level < P.nlevels) continue;
             GLAUNCH(N4_K_LEVEL, launch_level(gv, nvg, P.nlevels, R.prev_iters, rp, gs));
-            GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon, gs));
+            GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon,
+                                                 P.mt[P.nlevels - 1], P.nt[P.nlevels - 1], gs));
             for (int v = R.v0; v < R.v1; ++v)
                 if (in[v].out_lattice) {
                     const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
@@ -802,6 +836,14 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
 #endif
             R.finished = true;
             --remaining;
+        }
+        if (advanced) {
+            idle_rounds = 0;
+        } else {
+            ++idle_rounds;
+#if defined(__x86_64__) && !defined(N4_EMULATE)
+            __builtin_ia32_pause();
+#endif
         }
     }
     std::vector<VolState> st(nv);
@@ -922,9 +964,7 @@ n4_status n4_destroy(n4_handle* h) {
     dev_free(h->arena);
 #ifndef N4_EMULATE
     if (h->pinned_flags) (void)hipHostFree(h->pinned_flags);
-    if (h->flag_ev_init) {
-        for (int g = 0; g < kMaxGroups; ++g)
-            for (int k = 0; k < kFlagSlots; ++k) (void)hipEventDestroy(h->flag_ev[g][k]);
+    if (h->events_init) {
         (void)hipEventDestroy(h->fork_ev);
         for (int g = 0; g < kMaxGroups; ++g) (void)hipEventDestroy(h->join_ev[g]);
     }
@@ -1117,9 +1157,9 @@ n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_
     VolDesc* dd = dev_desc(h, P);
     Slab* sl = dev_wslab(h, P);
     LAUNCH(launch_level(dd, 1, 0, 0, P.rp, nullptr));
-    LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, kFitDen, P.rp, P.lds_fit_max[0], nullptr));
+    LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, 0, kFitDen, P.rp, P.lds_fit_max[0], P.mt[0], P.nt[0], nullptr));
     LAUNCH(launch_lattice(dd, 1, P.cx_max[0], 0, kFitDen, P.rp, nullptr));
-    LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, kFitField, P.rp, P.lds_fit_max[0], nullptr));
+    LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, 0, kFitField, P.rp, P.lds_fit_max[0], P.mt[0], P.nt[0], nullptr));
     LAUNCH(launch_lattice(dd, 1, P.cx_max[0], 0, kFitField, P.rp, nullptr));
     D2D(d_lattice, D.lresf, sizeof(float) * D.lev[0].L, nullptr);
     SYNC(nullptr);
@@ -1148,7 +1188,7 @@ n4_status n4_evaluate_bspline3d(n4_handle* h, const float* d_lattice, const uint
     D.latf = const_cast<float*>(d_lattice);
     memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
     H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
-    LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, nullptr));
+    LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, P.mt[0], P.nt[0], nullptr));
     SYNC(nullptr);
     return N4_OK;
 }

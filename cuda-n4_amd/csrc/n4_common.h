@@ -22,6 +22,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxLevels = 16;
 constexpr int kMaxBins = 512;
 constexpr int kChunkVox = 4096;   // working voxels staged in LDS per row chunk
+constexpr int kMfmaMaxTiles = 4;  // MFMA kernels: up to 4 x 16 control points per axis
 
 // One position along one axis at one level (A11/A12/A14 of the B-spline).
 struct AxisEntry {
@@ -75,12 +76,13 @@ struct VolDesc {
     float* latf;
     float* lresf;
     // partials
-    float* part;
+    float* part;               // per-slab fit contraction (DEN and FIELD modes)
+    uint64_t* numfx[2];        // fixed-point lattice numerator, by iteration parity
     uint64_t* hist;
     double* spart;
     double* etab;
     VolState* st;
-    int32_t* done_flag;        // mirror of st->done in a dense per-batch array
+    uint64_t* progress;        // progress word in pinned host memory (see progress_word)
     LevelDesc lev[kMaxLevels];
     const AxisEntry* fax[3];   // full-res axis tables for the final lattice
     int32_t nlevels;
@@ -110,6 +112,7 @@ struct RunParams {
     double low;       // low value threshold
     float low_f;      // largest float <= low: x > low_f  <=>  (double)x > low
     int32_t fixed_iters;
+    int32_t prec;     // n4_precision: 0 FP32, 1 BF16 MFMA, 2 split-BF16 MFMA
 };
 
 // One workgroup's tile: rows [y0, y0+ny) of slab x of volume vol.  The
@@ -121,10 +124,26 @@ struct Slab {
     int32_t ny;
 };
 
+// Progress word of one volume, written by the device into pinned host
+// memory (one 64-bit store, so the fields are always read consistently):
+// bits 40.. level + 1, bits 1..39 histogram passes run at that level
+// (pass it+1 runs only after iteration it-1 has finished), bit 0 done.
+constexpr uint64_t progress_word(int level, int passes, int done) {
+    return ((uint64_t)(level + 1) << 40) | ((uint64_t)passes << 1) | (uint64_t)(done ? 1 : 0);
+}
+constexpr int progress_level(uint64_t w) { return (int)(w >> 40) - 1; }
+constexpr int progress_passes(uint64_t w) { return (int)((w >> 1) & ((1ull << 39) - 1)); }
+constexpr int progress_done(uint64_t w) { return (int)(w & 1); }
+
 // Exact floor(v / d) for v * d < 2^40 as one 64-bit multiply and shift:
 // m = floor(2^40 / d) + 1 (computed on the host).
 constexpr int kDivShift = 40;
 
 enum FitMode : int32_t { kFitNum = 0, kFitDen = 1, kFitField = 2 };
+
+// Fixed-point scale of the lattice numerator accumulated by the fit's
+// integer atomics (order-independent, so bitwise reproducible): 2^-30
+// resolution, range +-8.6e9.
+constexpr double kNumScale = 1073741824.0;  // 2^30
 
 }  // namespace n4

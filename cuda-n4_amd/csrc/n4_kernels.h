@@ -24,10 +24,14 @@ size_t lds_prep();
 size_t lds_level();
 size_t lds_hist(int nbins);
 size_t lds_wiener(int P);
-size_t lds_fit(int nbins, const VolDesc& v, int level);
+// prec (RunParams::prec) selects the FP32 or the MFMA kernels; recon_rows =
+// full-resolution rows per recon workgroup.
+size_t lds_fit(int nbins, const VolDesc& v, int level, int prec);
 size_t lds_lattice();
-size_t lds_eval(const VolDesc& v, int level);
-size_t lds_recon(const VolDesc& v);
+size_t lds_eval(const VolDesc& v, int level, int prec);
+size_t lds_recon(const VolDesc& v, int recon_rows, int prec);
+// MFMA tiles (16 control points each) for c control points along an axis.
+int mfma_tiles(int c);
 size_t lds_flat(int nbins);
 
 // `stream` is a hipStream_t (void* keeps this header HIP-free).  Return 0
@@ -39,13 +43,16 @@ int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunPa
 int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
                 int32_t* dbg_idx, size_t lds, void* stream);
 int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void* stream);
-int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int mode, RunParams rp, size_t lds,
-               void* stream);
+// mt, nt: MFMA tiles along y and z of the level's lattice (used when
+// rp.prec != 0; the MFMA kernels exist for 1..4 tiles per axis).
+// it: iteration within the level (selects the numerator buffer by parity).
+int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, int mode, RunParams rp,
+               size_t lds, int mt, int nt, void* stream);
 int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mode, RunParams rp, void* stream);
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
-                void* stream);
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
-                 void* stream);
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp, size_t lds,
+                int mt, int nt, void* stream);
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, int mt,
+                 int nt, void* stream);
 
 // Flat (1-D) stage helpers used by the stage entry points.  A flat array of
 // n voxels is cut into slabs of kChunkVox voxels; each slab writes

@@ -84,6 +84,7 @@ HD void sincospi_d(double x, double* s, double* c) {
 }
 
 HD size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+HD int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // floor(v / d) with the host-computed magic m = floor(2^40/d) + 1
 // (exact while v * d < 2^40).
@@ -103,14 +104,15 @@ struct Carve {
     }
 };
 
-// Level-tagged done flag in pinned host memory (level+1 when the volume has
-// finished the level, 0 otherwise), read by the host's early-exit poll.
-HD void publish_done(const VolDesc& V, int level, int done) {
-    if (!V.done_flag) return;
+// Progress word in pinned host memory, read by the host's early-exit poll
+// (no events in the iteration stream).
+HD void publish_progress(const VolDesc& V, int level, int passes, int done) {
+    if (!V.progress) return;
+    const uint64_t w = progress_word(level, passes, done);
 #ifdef N4_DEVICE_PASS
-    __hip_atomic_store(V.done_flag, done ? level + 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(V.progress, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 #else
-    *V.done_flag = done ? level + 1 : 0;
+    *V.progress = w;
 #endif
 }
 
@@ -346,7 +348,15 @@ HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp,
         }
     }
     BLOCK_END
-    if (level >= V.nlevels) return;
+    if (level >= V.nlevels) {
+        // final lattice in FP32 for the reconstruction and the lattice output
+        const LevelDesc& FD = V.lev[V.nlevels - 1];
+        const double* fin = V.lat[(V.nlevels - 1) & 1];
+        BLOCK_FOR(t)
+        for (int l = t; l < FD.L; l += kBlock) V.latf[l] = (float)fin[l];
+        BLOCK_END
+        return;
+    }
     const LevelDesc& LD = V.lev[level];
     double* dst = V.lat[level & 1];
     if (level == 0) {
@@ -355,10 +365,18 @@ HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp,
             dst[l] = 0.0;
             V.latf[l] = 0.0f;
             V.lresf[l] = 0.0f;
+            V.numfx[0][l] = 0;
+            V.numfx[1][l] = 0;
         }
         for (int i = t; i < rp.nbins; i += kBlock) V.hist[i] = 0;
         BLOCK_END
     } else {
+        BLOCK_FOR(t)
+        for (int l = t; l < LD.L; l += kBlock) {
+            V.numfx[0][l] = 0;
+            V.numfx[1][l] = 0;
+        }
+        BLOCK_END
         const LevelDesc& PD = V.lev[level - 1];
         const double* src = V.lat[(level - 1) & 1];
         BLOCK_FOR(t)
@@ -404,7 +422,7 @@ HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp,
         st->iter = 0;
         st->cv = 1e10;
         st->done = (V.max_iter[level] <= 0) ? 1 : 0;
-        publish_done(V, level, st->done);
+        publish_progress(V, level, 0, st->done);
         st->iters[level] = 0;
         st->lastcv[level] = 1e10;
     }
@@ -448,11 +466,9 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, Run
                 st->iters[level] = it;
                 st->lastcv[level] = cvv;
                 st->cv = cvv;
-                if (stop) {
-                    st->done = 1;
-                    publish_done(V, level, 1);
-                }
+                if (stop) st->done = 1;
             }
+            publish_progress(V, level, it + 1, stop ? 1 : 0);
             st->iter = it;
             st->bin_min = mn;
             st->bin_max = mx;
@@ -673,6 +689,49 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
     BLOCK_END
 }
 
+// ---------------------------------------------------------- lattice update --
+// The fit of slab x adds its (y, z) contraction T2, weighted by the x
+// factor Px[x, cx], to the numerator of the 4 control planes cx = b(x) ..
+// b(x)+3 it touches, as fixed-point integer atomics into numfx[it & 1]
+// (integer sums do not depend on order, so the result is bitwise
+// reproducible): num[cx,p] = sum_x Px[x,cx] T2_x[p]  (N4.py:186-237).
+HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry& ex, int it, int pr, float t2) {
+    const int CyCz = LD.C[1] * LD.C[2];
+    uint64_t* acc = V.numfx[it & 1];
+    UNROLL for (int k = 0; k < 4; ++k) {
+        const double c = (double)ex.p[k] * (double)t2;
+        const long long q = (long long)rint(c * kNumScale);
+        global_add_u64(acc + (size_t)(ex.b + k) * CyCz + pr, (unsigned long long)q);
+    }
+}
+
+// Lattice increment of the 4 control planes slab x touches: lres =
+// num / (den + eps) (N4.py:236).  The block of the first slab of a plane's
+// support also adds lres to the FP64 lattice (N4.py:317) and clears the
+// plane in the other numerator buffer for the next iteration, so each plane
+// is updated exactly once; planes without support have num = 0.  Runs
+// inside a thread phase (thread t of kBlock).
+HD void lattice_planes(const VolDesc& V, const LevelDesc& LD, double* lat, int x, const AxisEntry& ex, int it,
+                       double eps, float* LR, int t) {
+    const int CyCz = LD.C[1] * LD.C[2];
+    const uint64_t* acc = V.numfx[it & 1];
+    uint64_t* next = V.numfx[(it + 1) & 1];
+    for (int k = 0; k < 4; ++k) {
+        const int cx = ex.b + k;
+        const bool owner = (x == LD.sup[0][2 * cx]);
+        for (int pr = t; pr < CyCz; pr += kBlock) {
+            const size_t l = (size_t)cx * CyCz + pr;
+            const double num = (double)(long long)acc[l] / kNumScale;
+            const double lres = num / (V.den[l] + eps);
+            if (owner) {
+                lat[l] += lres;
+                next[l] = 0;
+            }
+            LR[k * CyCz + pr] = (float)lres;
+        }
+    }
+}
+
 // ------------------------------------------------------------------- fit --
 // Remap (N4.py:103-115) + residual (N4.py:313) + the (y, z) part of the
 // separable B-spline fit (N4.py:186-237 as mode-n contractions, exact
@@ -695,6 +754,23 @@ struct FitLds {
     int* yb;
 };
 
+// Contracted quantity g of one working voxel: the residual logU - E[u|v]
+// (N4.py:103-115 remap with the cudaN4.cu:242 clamp, then N4.py:313) for
+// NUM, the mask for DEN, logU itself for FIELD; 0 outside the mask.
+HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const double* E, double bmin, double slope,
+                   int nb) {
+    if (!m) return 0.0f;
+    if (mode == kFitDen) return 1.0f;
+    if (mode == kFitField) return xv;
+    if (degenerate) return 0.0f;
+    double c = ((double)xv - bmin) / slope;
+    c = fmin(fmax(c, 0.0), (double)nb);
+    const double fl = floor(c);
+    const int idx = (int)fl;
+    const double sh = (idx < nb - 1) ? E[idx] + (E[idx + 1] - E[idx]) * (c - fl) : E[nb - 1];
+    return (float)((double)xv - sh);
+}
+
 HD size_t carve_fit(unsigned char* base, int nbins, int Z, int R, int Cy, int Cz, FitLds* o) {
     Carve c{base, 0};
     FitLds L;
@@ -712,7 +788,7 @@ HD size_t carve_fit(unsigned char* base, int nbins, int Z, int R, int Cy, int Cz
     return align16(c.off);
 }
 
-HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, RunParams rp, int bid,
+HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int mode, RunParams rp, int bid,
                  unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
@@ -765,24 +841,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
             UNROLL for (int u = 0; u < 4; ++u) {
                 const int v = v0 + u * kBlock + t;
                 if (v >= ny * Z) continue;
-                float gv = 0.0f;
-                if (ms[u]) {
-                    if (mode == kFitDen) {
-                        gv = 1.0f;
-                    } else if (mode == kFitField) {
-                        gv = xs[u];
-                    } else if (!degenerate) {
-                        double c = ((double)xs[u] - bmin) / slope;
-                        c = fmin(fmax(c, 0.0), (double)nb);  // cudaN4.cu:242 clamp
-                        const double fl = floor(c);
-                        const int idx = (int)fl;
-                        const double sh = (idx < nb - 1)
-                                              ? S.E[idx] + (S.E[idx + 1] - S.E[idx]) * (c - fl)
-                                              : S.E[nb - 1];
-                        gv = (float)((double)xs[u] - sh);
-                    }
-                }
-                S.g[v] = gv;
+                S.g[v] = fit_input(mode, ms[u], xs[u], degenerate, S.E, bmin, slope, nb);
             }
         }
         BLOCK_END
@@ -814,8 +873,14 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int mode, Ru
         BLOCK_END
         BLOCK_SYNC();
     }
+    const AxisEntry ex = LD.ax[0][x];
     BLOCK_FOR(t)
-    for (int pr = t; pr < Cy * Cz; pr += kBlock) V.part[(size_t)x * V.part_stride + pr] = S.T2[pr];
+    for (int pr = t; pr < Cy * Cz; pr += kBlock) {
+        if (num)
+            scatter_numerator(V, LD, ex, it, pr, S.T2[pr]);
+        else
+            V.part[(size_t)x * V.part_stride + pr] = S.T2[pr];
+    }
     BLOCK_END
 }
 
@@ -880,17 +945,17 @@ HD void body_lattice(const VolDesc* vols, int level, int mode, RunParams rp, int
 }
 
 // ------------------------------------------------------------------ eval --
-// New field on the working grid (N4.py:318), convergence sums (N4.py:121-126)
-// and logU update (N4.py:322), fused:
-//   A[cy,cz]  = sum_a wx[a] L[bx+a,cy,cz]       (also for the increment Lres)
-//   B[y,cz]   = sum_b wy[b] A[by+b,cz]
-//   f(y,z)    = sum_c wz[c] B[y,bz+c];  d = same with Lres = new - old field
-//   logU = imlog - f;  u = exp(old-new) - 1 = expm1(-d)
+// Field increment on the working grid, convergence sums (N4.py:121-126) and
+// logU update (N4.py:318, 322), fused.  The field is linear in the lattice,
+// so new field = old field + d with d the evaluation of the increment, and
+//   Ar[cy,cz] = sum_a wx[a] lres[bx+a,cy,cz]
+//   Br[y,cz]  = sum_b wy[b] Ar[by+b,cz]
+//   d(y,z)    = sum_c wz[c] Br[y,bz+c]
+//   logU -= d  (= imlog - new field);  u = exp(old-new) - 1 = expm1(-d)
 // Per slab: spart[x] = (sum u, sum u^2 in FP64, min, max of logU).
 struct EvalLds {
-    float* A;
+    float* LR;
     float* Ar;
-    float* B;
     float* Br;
     float* zw;
     int* zb;
@@ -900,9 +965,8 @@ struct EvalLds {
 HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds* o) {
     Carve c{base, 0};
     EvalLds L;
-    L.A = c.take<float>((size_t)Cy * Cz);
+    L.LR = c.take<float>((size_t)4 * Cy * Cz);
     L.Ar = c.take<float>((size_t)Cy * Cz);
-    L.B = c.take<float>((size_t)R * Cz);
     L.Br = c.take<float>((size_t)R * Cz);
     L.zw = c.take<float>((size_t)4 * Z);
     L.zb = c.take<int>(Z);
@@ -911,9 +975,8 @@ HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds*
     return align16(c.off);
 }
 
-HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams rp, int bid,
+HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, int it, RunParams rp, int bid,
                   unsigned char* smem) {
-    (void)rp;
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     if (V.st->done) return;
@@ -927,14 +990,13 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
     const int x = sl.x;
     const AxisEntry ex = LD.ax[0][x];
     BLOCK_FOR(t)
+    lattice_planes(V, LD, V.lat[level & 1], x, ex, it, rp.eps, S.LR, t);
+    BLOCK_END
+    BLOCK_SYNC();
+    BLOCK_FOR(t)
     for (int pr = t; pr < CyCz; pr += kBlock) {
-        float a = 0.0f, ar = 0.0f;
-        for (int k = 0; k < 4; ++k) {
-            const size_t li = (size_t)(ex.b + k) * CyCz + pr;
-            a += ex.w[k] * V.latf[li];
-            ar += ex.w[k] * V.lresf[li];
-        }
-        S.A[pr] = a;
+        float ar = 0.0f;
+        for (int k = 0; k < 4; ++k) ar += ex.w[k] * S.LR[k * CyCz + pr];
         S.Ar[pr] = ar;
     }
     for (int z = t; z < Z; z += kBlock) {
@@ -955,12 +1017,8 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
             const AxisEntry& e = LD.ax[1][y0 + r];
-            float b = 0.0f, br = 0.0f;
-            for (int k = 0; k < 4; ++k) {
-                b += e.w[k] * S.A[(e.b + k) * Cz + c];
-                br += e.w[k] * S.Ar[(e.b + k) * Cz + c];
-            }
-            S.B[pr] = b;
+            float br = 0.0f;
+            for (int k = 0; k < 4; ++k) br += e.w[k] * S.Ar[(e.b + k) * Cz + c];
             S.Br[pr] = br;
         }
         BLOCK_END
@@ -970,23 +1028,21 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, RunParams r
         double mn = S.red[2 * kBlock + t], mx = S.red[3 * kBlock + t];
         for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
             uint8_t ms[4];
-            float il[4];
+            float lu[4];
             UNROLL for (int u = 0; u < 4; ++u) {
                 const int v = v0 + u * kBlock + t;
                 const bool ok = v < ny * Z;
                 ms[u] = ok ? V.wmask[base + v] : 0;
-                il[u] = ok ? V.imlog[base + v] : 0.0f;
+                lu[u] = (ok && ms[u]) ? V.logU[base + v] : 0.0f;
             }
             UNROLL for (int u = 0; u < 4; ++u) {
                 if (!ms[u]) continue;
                 const int v = v0 + u * kBlock + t;
                 const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
                 const float* wz = S.zw + 4 * z;
-                const float* br = S.B + r * Cz + S.zb[z];
                 const float* brr = S.Br + r * Cz + S.zb[z];
-                const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
                 const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
-                const float lg = il[u] - f;
+                const float lg = lu[u] - d;
                 V.logU[base + v] = lg;
                 const double uu = (double)expm1f(-d);
                 s1 += uu;
@@ -1196,7 +1252,18 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
     BLOCK_END
 }
 
+}  // namespace n4
+
+#ifndef N4_EMULATE
+#include "n4_mfma.h"
+#endif
+
+namespace n4 {
+
 // ---------------------------------------------------------------- sizing --
+
+// MFMA tiles of 16 control points along one axis (0: not an MFMA launch).
+static int cp_tiles(int c) { return (c + 15) / 16; }
 
 size_t lds_prep() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_level() { return align16(4 * kBlock * sizeof(double)); }
@@ -1205,20 +1272,40 @@ size_t lds_wiener(int P) { return align16(4 * (size_t)P * sizeof(D2)); }
 size_t lds_lattice() { return align16(kBlock * sizeof(double)); }
 size_t lds_flat(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(double)); }
 
-size_t lds_fit(int nbins, const VolDesc& v, int level) {
+size_t lds_fit(int nbins, const VolDesc& v, int level, int prec) {
     const LevelDesc& L = v.lev[level];
+#ifndef N4_EMULATE
+    if (prec)
+        return carve_fit_mfma(nullptr, nbins, v.W[1], v.W[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec, nullptr);
+#endif
+    (void)prec;
     return carve_fit(nullptr, nbins, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
 }
 
-size_t lds_eval(const VolDesc& v, int level) {
+size_t lds_eval(const VolDesc& v, int level, int prec) {
     const LevelDesc& L = v.lev[level];
+#ifndef N4_EMULATE
+    if (prec)
+        return carve_eval_mfma(nullptr, v.W[1], v.W[2], L.C[1], L.C[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec,
+                               true, nullptr);
+#endif
+    (void)prec;
     return carve_eval(nullptr, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
 }
 
-size_t lds_recon(const VolDesc& v) {
+size_t lds_recon(const VolDesc& v, int rows, int prec) {
     const LevelDesc& L = v.lev[v.nlevels - 1];
+#ifndef N4_EMULATE
+    if (prec)
+        return carve_eval_mfma(nullptr, rows, v.F[2], L.C[1], L.C[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec,
+                               false, nullptr);
+#endif
+    (void)prec;
+    (void)rows;
     return carve_recon(nullptr, v.F[2], rows_per_chunk(v.F[1], v.F[2]), L.C[1], L.C[2], nullptr);
 }
+
+int mfma_tiles(int c) { return cp_tiles(c); }
 
 // --------------------------------------------------------------- launches --
 #ifndef N4_EMULATE
@@ -1246,10 +1333,10 @@ __global__ __launch_bounds__(kBlock) void k_wiener(const VolDesc* vols, RunParam
     body_wiener(vols, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_fit(const VolDesc* vols, const Slab* slabs, int level, int mode,
-                                                RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_fit(const VolDesc* vols, const Slab* slabs, int level, int it,
+                                                int mode, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_fit(vols, slabs, level, mode, rp, blockIdx.x, SMEM());
+    body_fit(vols, slabs, level, it, mode, rp, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* vols, int level, int mode, RunParams rp) {
@@ -1257,10 +1344,10 @@ __global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* vols, int lev
     body_lattice(vols, level, mode, rp, blockIdx.x, blockIdx.y, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, int level,
+__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, int level, int it,
                                                  RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_eval(vols, slabs, level, rp, blockIdx.x, SMEM());
+    body_eval(vols, slabs, level, it, rp, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_recon(const VolDesc* vols, const Slab* slabs, int level,
@@ -1273,6 +1360,62 @@ __global__ __launch_bounds__(kBlock) void k_flat(const VolDesc* vols, const Slab
                                                  const float* a, const float* b, float* out, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_flat(vols, slabs, op, a, b, out, rp, blockIdx.x, SMEM());
+}
+
+template <int PREC, int MT, int NT>
+__global__ __launch_bounds__(kBlock) void k_fit_mfma(const VolDesc* vols, const Slab* slabs, int level, int it,
+                                                     int mode, RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_fit_mfma<PREC, MT, NT>(vols, slabs, level, it, mode, rp, blockIdx.x, SMEM());
+}
+
+template <int PREC, int MT, int NT>
+__global__ __launch_bounds__(kBlock) void k_eval_mfma(const VolDesc* vols, const Slab* slabs, int level, int it,
+                                                      RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_eval_mfma<PREC, MT, NT>(vols, slabs, level, it, rp, blockIdx.x, SMEM());
+}
+
+template <int PREC, int MT, int NT>
+__global__ __launch_bounds__(kBlock) void k_recon_mfma(const VolDesc* vols, const Slab* slabs, int level,
+                                                       RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_recon_mfma<PREC, MT, NT>(vols, slabs, level, rp, blockIdx.x, SMEM());
+}
+
+// Launchers of every (precision, Cy tiles, Cz tiles) instance, indexed
+// [prec-1][MT-1][NT-1].
+typedef void (*FitLaunch)(int, size_t, hipStream_t, const VolDesc*, const Slab*, int, int, int, RunParams);
+typedef void (*EvalLaunch)(int, size_t, hipStream_t, const VolDesc*, const Slab*, int, int, RunParams);
+
+template <int PREC, int MT, int NT>
+static void fit_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int level, int it,
+                            int mode, RunParams rp) {
+    hipLaunchKernelGGL((k_fit_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, level, it, mode, rp);
+}
+template <int PREC, int MT, int NT>
+static void eval_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int level, int it,
+                             RunParams rp) {
+    hipLaunchKernelGGL((k_eval_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, level, it, rp);
+}
+template <int PREC, int MT, int NT>
+static void recon_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int level, int,
+                              RunParams rp) {
+    hipLaunchKernelGGL((k_recon_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, level, rp);
+}
+
+#define N4_ROW(F, P, M) {F<P, M, 1>, F<P, M, 2>, F<P, M, 3>, F<P, M, 4>}
+#define N4_TABLE(F) \
+    {{N4_ROW(F, 1, 1), N4_ROW(F, 1, 2), N4_ROW(F, 1, 3), N4_ROW(F, 1, 4)}, \
+     {N4_ROW(F, 2, 1), N4_ROW(F, 2, 2), N4_ROW(F, 2, 3), N4_ROW(F, 2, 4)}}
+static const FitLaunch kFitMfma[2][kMfmaMaxTiles][kMfmaMaxTiles] = N4_TABLE(fit_mfma_launch);
+static const EvalLaunch kEvalMfma[2][kMfmaMaxTiles][kMfmaMaxTiles] = N4_TABLE(eval_mfma_launch);
+static const EvalLaunch kReconMfma[2][kMfmaMaxTiles][kMfmaMaxTiles] = N4_TABLE(recon_mfma_launch);
+#undef N4_TABLE
+#undef N4_ROW
+
+static bool mfma_ok(const RunParams& rp, int mt, int nt) {
+    return (rp.prec == 1 || rp.prec == 2) && mt >= 1 && mt <= kMfmaMaxTiles && nt >= 1 && nt <= kMfmaMaxTiles;
 }
 
 static int last_launch_status() { return (int)hipGetLastError(); }
@@ -1304,10 +1447,16 @@ int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void*
     return last_launch_status();
 }
 
-int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int mode, RunParams rp, size_t lds,
-               void* stream) {
+int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, int mode, RunParams rp,
+               size_t lds, int mt, int nt, void* stream) {
     if (nslab <= 0) return 0;
-    hipLaunchKernelGGL(k_fit, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, mode, rp);
+    if (rp.prec) {
+        if (!mfma_ok(rp, mt, nt)) return (int)hipErrorInvalidValue;
+        kFitMfma[rp.prec - 1][mt - 1][nt - 1](nslab, lds, (hipStream_t)stream, vols, slabs, level, it, mode, rp);
+        return last_launch_status();
+    }
+    hipLaunchKernelGGL(k_fit, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, it, mode,
+                       rp);
     return last_launch_status();
 }
 
@@ -1318,16 +1467,26 @@ int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mod
     return last_launch_status();
 }
 
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
-                void* stream) {
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp, size_t lds,
+                int mt, int nt, void* stream) {
     if (nslab <= 0) return 0;
-    hipLaunchKernelGGL(k_eval, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, rp);
+    if (rp.prec) {
+        if (!mfma_ok(rp, mt, nt)) return (int)hipErrorInvalidValue;
+        kEvalMfma[rp.prec - 1][mt - 1][nt - 1](nslab, lds, (hipStream_t)stream, vols, slabs, level, it, rp);
+        return last_launch_status();
+    }
+    hipLaunchKernelGGL(k_eval, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, it, rp);
     return last_launch_status();
 }
 
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
-                 void* stream) {
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, int mt,
+                 int nt, void* stream) {
     if (nslab <= 0) return 0;
+    if (rp.prec) {
+        if (!mfma_ok(rp, mt, nt)) return (int)hipErrorInvalidValue;
+        kReconMfma[rp.prec - 1][mt - 1][nt - 1](nslab, lds, (hipStream_t)stream, vols, slabs, level, 0, rp);
+        return last_launch_status();
+    }
     hipLaunchKernelGGL(k_recon, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, rp);
     return last_launch_status();
 }
@@ -1373,10 +1532,11 @@ int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void*
     return 0;
 }
 
-int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int mode, RunParams rp, size_t lds,
-               void*) {
+int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, int mode, RunParams rp,
+               size_t lds, int, int, void*) {
+    if (rp.prec) return 1;
     unsigned char* b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_fit(vols, slabs, level, mode, rp, i, b);
+    for (int i = 0; i < nslab; ++i) body_fit(vols, slabs, level, it, mode, rp, i, b);
     return 0;
 }
 
@@ -1387,13 +1547,17 @@ int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mod
     return 0;
 }
 
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, void*) {
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp, size_t lds,
+                int, int, void*) {
+    if (rp.prec) return 1;
     unsigned char* b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_eval(vols, slabs, level, rp, i, b);
+    for (int i = 0; i < nslab; ++i) body_eval(vols, slabs, level, it, rp, i, b);
     return 0;
 }
 
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, void*) {
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, int, int,
+                 void*) {
+    if (rp.prec) return 1;
     unsigned char* b = emu_lds(lds);
     for (int i = 0; i < nslab; ++i) body_recon(vols, slabs, level, rp, i, b);
     return 0;

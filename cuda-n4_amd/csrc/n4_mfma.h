@@ -1,0 +1,448 @@
+// n4_mfma.h -- BF16 and split-BF16 MFMA forms of the separable B-spline
+// contractions (device build only; included by n4_kernels.hip).
+//
+// The per-voxel B-spline normaliser factorises per axis, so the fit
+// numerator/denominator and the field evaluation are chains of mode-n
+// products with banded per-axis factor matrices (weights w, w^3/sum w^2,
+// w^2 of the axis tables).  Here each chain runs on gfx950 matrix cores:
+//
+//   fit   T1[y,cz]  = sum_z g[y,z] Pz[z,cz]          v_mfma_f32_16x16x32_bf16
+//         T2[cy,cz] = sum_y Py[y,cy] T1[y,cz]        v_mfma_f32_16x16x16_bf16
+//   eval  Bt[c,y]   = sum_cy A[cy,c] Wy[y,cy]        v_mfma_f32_16x16x16_bf16
+//         f[z,y]    = sum_c Wz[z,c] Bt[c,y]          v_mfma_f32_16x16x16_bf16
+//
+// The accumulator layout of one product is the operand layout of the next
+// (lane l holds rows 4*(l/16)+i of column l%16), so the chains never leave
+// registers.  Inputs are rounded to BF16 (PREC 1) or split into hi + lo BF16
+// halves with the product hi*hi + hi*lo + lo*hi (PREC 2); accumulation is
+// FP32 throughout.  The voxel-side work (remap, masks, exp, FP64 CV sums)
+// is identical to the FP32 kernels.
+//
+// A wave owns 16 rows (y) of the slab at a time; 4 waves per workgroup.
+// Control-point counts per axis are padded to 16 (MT/NT tiles of 16).
+#pragma once
+
+namespace n4 {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// float -> BF16 bits, round to nearest even (finite inputs).
+__device__ inline unsigned short bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_to_f32(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+// f ~= hi + lo with both halves BF16.
+__device__ inline void bf16_split(float f, unsigned short& hi, unsigned short& lo) {
+    hi = bf16_rn(f);
+    lo = bf16_rn(f - bf16_to_f32(hi));
+}
+
+__device__ inline f32x4 mma_k32(u16x8 a, u16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+__device__ inline f32x4 mma_k16(i16x4 a, i16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// c += a*b with PREC 1 (hi only) or PREC 2 (hi*hi + lo*hi + hi*lo).
+template <int PREC>
+__device__ inline f32x4 mma_k32p(u16x8 ah, u16x8 al, u16x8 bh, u16x8 bl, f32x4 c) {
+    if constexpr (PREC == 2) {
+        c = mma_k32(al, bh, c);
+        c = mma_k32(ah, bl, c);
+    }
+    return mma_k32(ah, bh, c);
+}
+template <int PREC>
+__device__ inline f32x4 mma_k16p(i16x4 ah, i16x4 al, i16x4 bh, i16x4 bl, f32x4 c) {
+    if constexpr (PREC == 2) {
+        c = mma_k16(al, bh, c);
+        c = mma_k16(ah, bl, c);
+    }
+    return mma_k16(ah, bh, c);
+}
+
+// Four accumulator floats as a K=16 operand (hi / lo).
+template <int PREC>
+__device__ inline void to_op4(f32x4 v, i16x4& hi, i16x4& lo) {
+    UNROLL for (int i = 0; i < 4; ++i) {
+        unsigned short h, l = 0;
+        if constexpr (PREC == 2)
+            bf16_split(v[i], h, l);
+        else
+            h = bf16_rn(v[i]);
+        hi[i] = (short)h;
+        lo[i] = (short)l;
+    }
+}
+
+// BF16 factor table in LDS: rows of `stride` halves, hi then (PREC 2) lo.
+struct Tab {
+    unsigned short* hi;
+    unsigned short* lo;
+    int stride;
+    __device__ void put(int r, int c, float v) const {
+        unsigned short h, l;
+        bf16_split(v, h, l);
+        hi[r * stride + c] = h;
+        if (lo) lo[r * stride + c] = l;
+    }
+    __device__ i16x4 get4(const unsigned short* t, int r, int c) const {
+        return *reinterpret_cast<const i16x4*>(t + r * stride + c);
+    }
+    __device__ u16x8 get8(const unsigned short* t, int r, int c) const {
+        return *reinterpret_cast<const u16x8*>(t + r * stride + c);
+    }
+};
+
+// One position of an axis (ep; nullptr for padding) in a factor table: 0
+// for every control point except its 4-point support, whose factors are w
+// (k 0), w^3/sum w^2 (k 1) or w^2 (k 2).  cp_rows: control points index the
+// table's rows (else its columns).
+__device__ inline void fill_axis(const Tab& T, bool cp_rows, int pos, int ncp_pad, int C, const AxisEntry* ep,
+                                 int k) {
+    for (int c = 0; c < ncp_pad; ++c) {
+        if (cp_rows)
+            T.put(c, pos, 0.0f);
+        else
+            T.put(pos, c, 0.0f);
+    }
+    if (!ep) return;
+    const AxisEntry e = *ep;
+    UNROLL for (int a = 0; a < 4; ++a) {
+        const int c = e.b + a;
+        const float v = k == 0 ? e.w[a] : (k == 1 ? e.p[a] : e.q[a]);
+        if (c >= 0 && c < C) {
+            if (cp_rows)
+                T.put(c, pos, v);
+            else
+                T.put(pos, c, v);
+        }
+    }
+}
+
+// ------------------------------------------------------------------- fit --
+struct FitMfmaLds {
+    double* E;
+    Tab Pz;   // [NT*16][Zp + 8]  Pz^T: row = control point, column = z
+    Tab Py;   // [MT*16][Yp + 4]  Py^T: row = control point, column = y
+    float* red;
+};
+
+HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
+    Carve c{base, 0};
+    FitMfmaLds L;
+    L.E = c.take<double>(nbins);
+    const int zs = round_up(Z, 32) + 8, ys = round_up(Y, 16) + 4;
+    L.Pz.stride = zs;
+    L.Pz.hi = c.take<unsigned short>((size_t)NT * 16 * zs);
+    L.Pz.lo = prec == 2 ? c.take<unsigned short>((size_t)NT * 16 * zs) : nullptr;
+    L.Py.stride = ys;
+    L.Py.hi = c.take<unsigned short>((size_t)MT * 16 * ys);
+    L.Py.lo = prec == 2 ? c.take<unsigned short>((size_t)MT * 16 * ys) : nullptr;
+    L.red = c.take<float>((size_t)MT * NT * 256);
+    if (o) *o = L;
+    return align16(c.off);
+}
+
+template <int PREC, int MT, int NT>
+__device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level, int it, int mode, RunParams rp,
+                              int bid, unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    const VolState* st = V.st;
+    if (mode == kFitNum && st->done) return;
+    const LevelDesc& LD = V.lev[level];
+    const int Y = V.W[1], Z = V.W[2];
+    const int Cy = LD.C[1], Cz = LD.C[2];
+    const int nb = rp.nbins;
+    FitMfmaLds S;
+    carve_fit_mfma(smem, nb, Y, Z, MT, NT, PREC, &S);
+    const bool num = (mode == kFitNum);
+    const bool degenerate = num && st->degenerate;
+    const double bmin = st->bin_min, slope = st->slope;
+    const int kf = (mode == kFitDen) ? 2 : 1;  // q = w^2 for DEN, p = w^3/S otherwise
+    const int x = sl.x;
+    const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
+    const int Zp = round_up(Z, 32), Yp = round_up(Y, 16);
+
+    if (num)
+        for (int i = t; i < nb; i += kBlock) S.E[i] = V.etab[i];
+    for (int z = t; z < Zp; z += kBlock) fill_axis(S.Pz, true, z, NT * 16, Cz, z < Z ? LD.ax[2] + z : nullptr, kf);
+    for (int y = t; y < Yp; y += kBlock) fill_axis(S.Py, true, y, MT * 16, Cy, y < Y ? LD.ax[1] + y : nullptr, kf);
+    __syncthreads();
+
+    f32x4 acc[MT][NT];
+    UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0, 0, 0, 0};
+
+    for (int y0 = 16 * wv; y0 < Y; y0 += 64) {
+        const int y = y0 + r16;
+        const bool yok = y < Y;
+        const size_t row = ((size_t)x * Y + (yok ? y : 0)) * Z;
+        f32x4 t1[NT];
+        UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
+        for (int z0 = 0; z0 < Z; z0 += 32) {
+            const int zb = z0 + 8 * g4;
+            uint8_t ms[8];
+            float xs[8];
+            UNROLL for (int j = 0; j < 8; ++j) {
+                const bool ok = yok && (zb + j) < Z;
+                ms[j] = ok ? V.wmask[row + zb + j] : 0;
+                xs[j] = (ok && mode != kFitDen) ? V.logU[row + zb + j] : 0.0f;
+            }
+            u16x8 ah, al;
+            UNROLL for (int j = 0; j < 8; ++j) {
+                const float gv = fit_input(mode, ms[j], xs[j], degenerate, S.E, bmin, slope, nb);
+                unsigned short h, l = 0;
+                if constexpr (PREC == 2)
+                    bf16_split(gv, h, l);
+                else
+                    h = bf16_rn(gv);
+                ah[j] = h;
+                al[j] = l;
+            }
+            UNROLL for (int n = 0; n < NT; ++n) {
+                const u16x8 bh = S.Pz.get8(S.Pz.hi, n * 16 + r16, zb);
+                const u16x8 bl = (PREC == 2) ? S.Pz.get8(S.Pz.lo, n * 16 + r16, zb) : bh;
+                t1[n] = mma_k32p<PREC>(ah, al, bh, bl, t1[n]);
+            }
+        }
+        // t1[n]: lane holds T1[y0 + 4*g4 + i][16n + r16] = the K=16 operand (k = y)
+        UNROLL for (int n = 0; n < NT; ++n) {
+            i16x4 bh, bl;
+            to_op4<PREC>(t1[n], bh, bl);
+            UNROLL for (int m = 0; m < MT; ++m) {
+                const i16x4 a_h = S.Py.get4(S.Py.hi, m * 16 + r16, y0 + 4 * g4);
+                const i16x4 a_l = (PREC == 2) ? S.Py.get4(S.Py.lo, m * 16 + r16, y0 + 4 * g4) : a_h;
+                acc[m][n] = mma_k16p<PREC>(a_h, a_l, bh, bl, acc[m][n]);
+            }
+        }
+    }
+    // waves combine in a fixed order (bitwise reproducible)
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+            UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) UNROLL for (int i = 0; i < 4; ++i) {
+                const int idx = (((m * NT + n) * 4 + i) << 6) + lane;
+                S.red[idx] = (w == 0 ? 0.0f : S.red[idx]) + acc[m][n][i];
+            }
+        }
+        __syncthreads();
+    }
+    float* out = V.part + (size_t)x * V.part_stride;
+    const AxisEntry ex = LD.ax[0][x];
+    for (int e = t; e < MT * NT * 256; e += kBlock) {
+        const int ln = e & 63, i = (e >> 6) & 3, tile = e >> 8;
+        const int m = tile / NT, n = tile - (tile / NT) * NT;
+        const int cy = m * 16 + 4 * (ln >> 4) + i, cz = n * 16 + (ln & 15);
+        if (cy < Cy && cz < Cz) {
+            if (num)
+                scatter_numerator(V, LD, ex, it, cy * Cz + cz, S.red[e]);
+            else
+                out[cy * Cz + cz] = S.red[e];
+        }
+    }
+}
+
+// ------------------------------------------------------- eval and recon --
+// Shared by the working-grid evaluation of the lattice increment (logU
+// update and CV sums) and the full-resolution reconstruction of the final
+// field (outputs).  Four control planes along x feed each slab:
+// A[cy,cz] = sum_k wx[k] plane_k[cy,cz] (FP32), then the (y, z) chain runs on
+// MFMA.
+struct EvalMfmaLds {
+    float* LR;   // [4][Cy*Cz]  lattice increment planes (eval only)
+    float* A;    // [Cy*Cz]     x-contracted planes
+    Tab At;      // [NT*16][MT*16 + 4]  A^T: row = cz, column = cy
+    Tab Wy;      // [Yrows][MT*16 + 4]  row = y (slab-local), column = cy
+    Tab Wz;      // [Zp16][NT*16 + 4]   row = z, column = cz
+    double* red; // [4*kBlock]  (eval only)
+};
+
+HD size_t carve_eval_mfma(unsigned char* base, int Yrows, int Z, int Cy, int Cz, int MT, int NT, int prec,
+                          bool eval, EvalMfmaLds* o) {
+    Carve c{base, 0};
+    EvalMfmaLds L;
+    L.LR = eval ? c.take<float>((size_t)4 * Cy * Cz) : nullptr;
+    L.A = c.take<float>((size_t)Cy * Cz);
+    const int as = MT * 16 + 4, zs = NT * 16 + 4;
+    L.At.stride = as;
+    L.At.hi = c.take<unsigned short>((size_t)NT * 16 * as);
+    L.At.lo = prec == 2 ? c.take<unsigned short>((size_t)NT * 16 * as) : nullptr;
+    const int yr = round_up(Yrows, 16);
+    L.Wy.stride = as;
+    L.Wy.hi = c.take<unsigned short>((size_t)yr * as);
+    L.Wy.lo = prec == 2 ? c.take<unsigned short>((size_t)yr * as) : nullptr;
+    const int zr = round_up(Z, 16);
+    L.Wz.stride = zs;
+    L.Wz.hi = c.take<unsigned short>((size_t)zr * zs);
+    L.Wz.lo = prec == 2 ? c.take<unsigned short>((size_t)zr * zs) : nullptr;
+    L.red = eval ? c.take<double>((size_t)4 * kBlock) : nullptr;
+    if (o) *o = L;
+    return align16(c.off);
+}
+
+// A from the 4 planes (plane k at planes + k*Cy*Cz) and the BF16 tables.
+// Rows [ylo, ylo + yrows) of axis table ay are the slab's rows.
+template <int MT, int NT>
+__device__ void eval_mfma_setup(const EvalMfmaLds& S, const AxisEntry ex, const float* planes, const AxisEntry* ay,
+                                int ylo, int yrows, const AxisEntry* az, int Z, int Cy, int Cz) {
+    const int t = (int)threadIdx.x;
+    const int CyCz = Cy * Cz;
+    for (int pr = t; pr < CyCz; pr += kBlock) {
+        float a = 0.0f;
+        UNROLL for (int k = 0; k < 4; ++k) a += ex.w[k] * planes[(size_t)k * CyCz + pr];
+        S.A[pr] = a;
+    }
+    const int yr = round_up(yrows, 16);
+    for (int r = t; r < yr; r += kBlock) fill_axis(S.Wy, false, r, MT * 16, Cy, r < yrows ? ay + ylo + r : nullptr, 0);
+    const int zr = round_up(Z, 16);
+    for (int z = t; z < zr; z += kBlock) fill_axis(S.Wz, false, z, NT * 16, Cz, z < Z ? az + z : nullptr, 0);
+    __syncthreads();
+    for (int e = t; e < NT * 16 * MT * 16; e += kBlock) {
+        const int cz = e / (MT * 16), cy = e - cz * (MT * 16);
+        S.At.put(cz, cy, (cz < Cz && cy < Cy) ? S.A[cy * Cz + cz] : 0.0f);
+    }
+    __syncthreads();
+}
+
+// Bt[n] operands for the 16 rows starting at slab-local row r0:
+// lane holds Bt[c = 16n + 4*g4 + i][y = r0 + r16] as a K=16 operand.
+template <int PREC, int MT, int NT>
+__device__ inline void eval_mfma_rows(const EvalMfmaLds& S, int r0, int r16, int g4, i16x4 (&bh)[NT],
+                                      i16x4 (&bl)[NT]) {
+    UNROLL for (int n = 0; n < NT; ++n) {
+        f32x4 b = f32x4{0, 0, 0, 0};
+        UNROLL for (int k = 0; k < MT; ++k) {
+            const i16x4 a_h = S.At.get4(S.At.hi, n * 16 + r16, k * 16 + 4 * g4);
+            const i16x4 a_l = (PREC == 2) ? S.At.get4(S.At.lo, n * 16 + r16, k * 16 + 4 * g4) : a_h;
+            const i16x4 w_h = S.Wy.get4(S.Wy.hi, r0 + r16, k * 16 + 4 * g4);
+            const i16x4 w_l = (PREC == 2) ? S.Wy.get4(S.Wy.lo, r0 + r16, k * 16 + 4 * g4) : w_h;
+            b = mma_k16p<PREC>(a_h, a_l, w_h, w_l, b);
+        }
+        to_op4<PREC>(b, bh[n], bl[n]);
+    }
+}
+
+// f[z = z0 + 4*g4 + i][y = r16 row] for one 16x16 (z, y) tile.
+template <int PREC, int NT>
+__device__ inline f32x4 eval_mfma_tile(const EvalMfmaLds& S, int z0, int r16, int g4, const i16x4 (&bh)[NT],
+                                       const i16x4 (&bl)[NT]) {
+    f32x4 f = f32x4{0, 0, 0, 0};
+    UNROLL for (int n = 0; n < NT; ++n) {
+        const i16x4 w_h = S.Wz.get4(S.Wz.hi, z0 + r16, n * 16 + 4 * g4);
+        const i16x4 w_l = (PREC == 2) ? S.Wz.get4(S.Wz.lo, z0 + r16, n * 16 + 4 * g4) : w_h;
+        f = mma_k16p<PREC>(w_h, w_l, bh[n], bl[n], f);
+    }
+    return f;
+}
+
+// Field increment d on the working grid, logU -= d, CV sums and logU
+// min/max per slab (same contract as the FP32 evaluation kernel).
+template <int PREC, int MT, int NT>
+__device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, int level, int it, RunParams rp, int bid,
+                               unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    if (V.st->done) return;
+    const LevelDesc& LD = V.lev[level];
+    const int Y = V.W[1], Z = V.W[2];
+    const int Cy = LD.C[1], Cz = LD.C[2];
+    EvalMfmaLds S;
+    carve_eval_mfma(smem, Y, Z, Cy, Cz, MT, NT, PREC, true, &S);
+    const int x = sl.x;
+    const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
+    const AxisEntry ex = LD.ax[0][x];
+    lattice_planes(V, LD, V.lat[level & 1], x, ex, it, rp.eps, S.LR, t);
+    __syncthreads();
+    eval_mfma_setup<MT, NT>(S, ex, S.LR, LD.ax[1], 0, Y, LD.ax[2], Z, Cy, Cz);
+    double s1 = 0.0, s2 = 0.0, mn = INFINITY, mx = -INFINITY;
+    for (int y0 = 16 * wv; y0 < Y; y0 += 64) {
+        i16x4 bh[NT], bl[NT];
+        eval_mfma_rows<PREC, MT, NT>(S, y0, r16, g4, bh, bl);
+        const int y = y0 + r16;
+        const bool yok = y < Y;
+        const size_t row = ((size_t)x * Y + (yok ? y : 0)) * Z;
+        for (int z0 = 0; z0 < Z; z0 += 16) {
+            const f32x4 d = eval_mfma_tile<PREC, NT>(S, z0, r16, g4, bh, bl);
+            const int zb = z0 + 4 * g4;
+            uint8_t ms[4];
+            float lu[4];
+            UNROLL for (int i = 0; i < 4; ++i) {
+                const bool ok = yok && (zb + i) < Z;
+                ms[i] = ok ? V.wmask[row + zb + i] : 0;
+                lu[i] = (ok && ms[i]) ? V.logU[row + zb + i] : 0.0f;
+            }
+            UNROLL for (int i = 0; i < 4; ++i) {
+                if (!ms[i]) continue;
+                const float lg = lu[i] - d[i];
+                V.logU[row + zb + i] = lg;
+                const double uu = (double)expm1f(-d[i]);
+                s1 += uu;
+                s2 += uu * uu;
+                mn = fmin(mn, (double)lg);
+                mx = fmax(mx, (double)lg);
+            }
+        }
+    }
+    S.red[t] = s1;
+    S.red[kBlock + t] = s2;
+    S.red[2 * kBlock + t] = mn;
+    S.red[3 * kBlock + t] = mx;
+    tree_reduce4(S.red, 0, 0, 1, 2);
+    if (t == 0) {
+        double* sp = V.spart + (size_t)x * 4;
+        sp[0] = S.red[0];
+        sp[1] = S.red[kBlock];
+        sp[2] = S.red[2 * kBlock];
+        sp[3] = S.red[3 * kBlock];
+    }
+}
+
+template <int PREC, int MT, int NT>
+__device__ void body_recon_mfma(const VolDesc* vols, const Slab* slabs, int level, RunParams rp, int bid,
+                                unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    const LevelDesc& LD = V.lev[level];
+    const int Y = V.F[1], Z = V.F[2];
+    const int Cy = LD.C[1], Cz = LD.C[2];
+    EvalMfmaLds S;
+    carve_eval_mfma(smem, sl.ny, Z, Cy, Cz, MT, NT, PREC, false, &S);
+    const int x = sl.x;
+    const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
+    const AxisEntry ex = V.fax[0][x];
+    eval_mfma_setup<MT, NT>(S, ex, V.latf + (size_t)ex.b * Cy * Cz, V.fax[1], sl.y0, sl.ny, V.fax[2], Z, Cy, Cz);
+    for (int r0 = 16 * wv; r0 < sl.ny; r0 += 64) {
+        i16x4 bh[NT], bl[NT];
+        eval_mfma_rows<PREC, MT, NT>(S, r0, r16, g4, bh, bl);
+        const int r = r0 + r16;
+        const bool yok = r < sl.ny;
+        const size_t row = ((size_t)x * Y + sl.y0 + (yok ? r : 0)) * Z;
+        for (int z0 = 0; z0 < Z; z0 += 16) {
+            const f32x4 f = eval_mfma_tile<PREC, NT>(S, z0, r16, g4, bh, bl);
+            const int zb = z0 + 4 * g4;
+            float xin[4];
+            UNROLL for (int i = 0; i < 4; ++i)
+                xin[i] = (V.out_norm && yok && (zb + i) < Z) ? V.im[row + zb + i] : 0.0f;
+            UNROLL for (int i = 0; i < 4; ++i) {
+                if (!yok || zb + i >= Z) continue;
+                const size_t v = row + zb + i;
+                if (V.out_logbias) V.out_logbias[v] = f[i];
+                if (V.out_norm || V.out_bias) {
+                    const float ef = expf(f[i]);
+                    if (V.out_bias) V.out_bias[v] = ef;
+                    if (V.out_norm) V.out_norm[v] = ((xin[i] <= rp.low_f) ? 0.0f : xin[i]) / ef;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace n4

@@ -5,8 +5,8 @@
 // axis order flipped (fx = slowest file axis, main.cpp:153-155), write the
 // outputs.  Extensions (the parameter surface N4.py and N4Param expose but
 // the reference CLI does not): -s shrink, --bins, --fwhm, --noise,
-// --threshold, --low, --iters a,b,c (per level), --eps, --lattice file,
-// --report file.json.  Volumes are NIfTI-1 (.nii/.nii.gz), read and written
+// --threshold, --low, --iters a,b,c (per level), --eps, --precision,
+// --lattice file, --report file.json.  Volumes are NIfTI-1 (.nii/.nii.gz), read and written
 // with the input's header (orientation kept).
 #include <errno.h>
 #include <stdio.h>
@@ -39,6 +39,7 @@ int usage() {
     printf("  -s shrink                  : fit on a grid shrunk by this factor per axis (default 1)\n");
     printf("  --bins N --fwhm F --noise F --threshold F --low F --eps F\n");
     printf("  --iters a,b,c              : maximum iterations per level\n");
+    printf("  --precision fp32|bf16|split_bf16 : B-spline contractions (default fp32)\n");
     printf("  --lattice out.nii.gz       : write the final B-spline lattice\n");
     printf("  --report out.json          : write a JSON run record\n");
     return -1;
@@ -141,6 +142,17 @@ int run(int argc, char** argv) {
                 param.IterationsPerLevel.push_back((unsigned)strtoul(s.substr(pos, e - pos).c_str(), nullptr, 10));
                 pos = e + 1;
             }
+        } else if (arg == "--precision") {
+            const std::string v = cl.next();
+            if (v == "fp32")
+                param.Precision = N4_PREC_FP32;
+            else if (v == "bf16")
+                param.Precision = N4_PREC_BF16;
+            else if (v == "split_bf16")
+                param.Precision = N4_PREC_SPLIT_BF16;
+            else
+                throw std::runtime_error("Expected fp32, bf16 or split_bf16 as parameter to '--precision', instead got '" +
+                                         v + "'");
         } else if (arg == "--lattice") {
             lattice_file = cl.next();
         } else if (arg == "--report") {

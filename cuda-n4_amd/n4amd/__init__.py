@@ -24,6 +24,7 @@ EMU_PATH = os.path.join(BUILD_DIR, "libn4emu.so")
 MAX_LEVELS = 16
 MASK_F32, MASK_U8, MASK_NONE = 0, 1, 2
 PREC_FP32, PREC_BF16, PREC_SPLIT_BF16 = 0, 1, 2
+PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "split_bf16": PREC_SPLIT_BF16}
 FLAG_FIXED_ITERATIONS = 0x1
 
 # Names every build of the library must export (include/n4_capi.h).
@@ -152,7 +153,8 @@ def load(emulate: bool = False) -> "Library":
 def default_params(**kw) -> Params:
     """n4_params with the reference defaults; keyword overrides:
     nbins, noise, fwhm, ncpt (int or 3-tuple), levels, max_iter (int or list),
-    threshold, low_value, shrink, eps, precision, fixed_iterations."""
+    threshold, low_value, shrink, eps, precision ("fp32", "bf16",
+    "split_bf16" or the n4_precision value), fixed_iterations."""
     p = Params()
     p.number_of_histogram_bins = 200
     p.wiener_filter_noise = 0.01
@@ -180,6 +182,8 @@ def default_params(**kw) -> Params:
             v = [v] * MAX_LEVELS if np.isscalar(v) else list(v)
             for i in range(MAX_LEVELS):
                 p.max_iterations[i] = int(v[i]) if i < len(v) else int(v[-1])
+        elif k == "precision":
+            p.precision = PRECISIONS[v] if isinstance(v, str) else int(v)
         elif k == "fixed_iterations":
             p.flags = (p.flags | FLAG_FIXED_ITERATIONS) if v else (p.flags & ~FLAG_FIXED_ITERATIONS)
         else:

@@ -50,3 +50,13 @@ def test_batch_equals_single(emu):
     for (im, m), r in zip(vols, rb):
         s = emu.correct(im, m, levels=2, max_iter=[4, 4])
         assert np.array_equal(r["log_bias"], s["log_bias"])
+
+
+def test_mfma_precisions_need_the_device_build(emu):
+    """The host emulation compiles the FP32 kernel bodies only; the MFMA
+    precision modes are refused there with N4_ERR_UNSUPPORTED."""
+    import n4amd
+    im, m, _ = PC.small_phantom()
+    for prec in ("bf16", "split_bf16"):
+        with pytest.raises(n4amd.N4Error, match="FP32 kernels only"):
+            emu.correct(im, m, levels=2, max_iter=[2, 2], precision=prec)
