@@ -236,6 +236,7 @@ struct Plan {
 
 static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
 
+
 // Magic multiplier for floor(v / d) (see fdiv in the kernels).
 static uint64_t magic_div(uint32_t d) { return ((uint64_t)1 << kDivShift) / d + 1; }
 
@@ -282,8 +283,6 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         I.prep_rows = rows_for(I.W[1], I.W[2], kPrepTileVox);
         I.prep_tiles_x = (I.W[1] + I.prep_rows - 1) / I.prep_rows;
         I.recon_rows = rows_for(in[v].F[1], in[v].F[2], kReconTileVox);
-        // MFMA recon: four waves of 16 rows each per workgroup
-        if (P->rp.prec) I.recon_rows = std::min(in[v].F[1], (I.recon_rows + 63) / 64 * 64);
         I.recon_tiles_x = (in[v].F[1] + I.recon_rows - 1) / I.recon_rows;
         nws += I.W[0];
         nps += I.W[0] * I.prep_tiles_x;
@@ -444,7 +443,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             P->mt[l] = mfma_tiles(D.lev[l].C[1]);
             P->nt[l] = mfma_tiles(D.lev[l].C[2]);
         }
-        if (need_tables_full) P->lds_recon = std::max(P->lds_recon, lds_recon(D, I.recon_rows, prec));
+        if (need_tables_full) P->lds_recon = std::max(P->lds_recon, lds_recon(D));
     }
     P->lds_hist = lds_hist(nb);
     P->lds_wiener = lds_wiener(P->rp.P);
@@ -644,14 +643,14 @@ static n4_status check_progress(n4_handle* h, int v0, int v1, int level, int it,
     return N4_OK;
 }
 
-// Volume groups of a batch: each group runs its own level/iteration chain
+// Volume groups of a batch: each group runs its own chain of steps
 // on its own stream (group 0 on the caller's stream), so the small
 // per-volume kernels of one group overlap the slab kernels of the others and
 // a group whose volumes converge stops without waiting for the rest.
 // Volumes are dealt to groups largest first (working voxels), each to the
 // group with the least work so far.
 static int group_count(int nvol) {
-    int g = nvol >= 4 ? 2 : 1;
+    int g = nvol >= 8 ? 4 : (nvol >= 4 ? 2 : 1);
     if (const char* e = getenv("N4_GROUPS")) {
         const int v = atoi(e);
         if (v >= 1) g = v;
@@ -665,9 +664,10 @@ struct GroupRun {
     int v0 = 0, v1 = 0;    // volumes
     int ws0 = 0, ws1 = 0;  // working slabs
     int fs0 = 0, fs1 = 0;  // recon tiles
+    int ps0 = 0, ps1 = 0;  // prep tiles
     void* stream = nullptr;
-    int level = 0, it = 0, prev_iters = 0;
-    bool started = false, finished = false;
+    int it = 0;            // steps enqueued
+    bool finished = false;
 };
 
 #define GLAUNCH(id, expr)                  \
@@ -742,27 +742,27 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     Slab* wsl = dev_wslab(h, P);
     Slab* fsl = dev_fslab(h, P);
     const RunParams rp = P.rp;
-    {
-        void* gs = stream;
-        GLAUNCH(N4_K_PREP, launch_prep(dd, dev_pslab(h, P), P.n_pslab, 0, rp, gs));
-    }
-    // group ranges (volumes, working slabs and recon tiles are contiguous)
+    // group ranges (volumes and their working slabs, prep tiles and recon
+    // tiles are contiguous)
     std::vector<GroupRun> gr(G);
     {
-        int v = 0, ws = 0, fs = 0;
+        int v = 0, ws = 0, fs = 0, ps = 0;
         for (int g = 0; g < G; ++g) {
             GroupRun& R = gr[g];
             R.v0 = v;
             R.ws0 = ws;
             R.fs0 = fs;
+            R.ps0 = ps;
             while (v < nv && gid[order[v]] == g) {
                 ws += P.desc[v].W[0];
                 fs += P.recon_tiles[v];
+                ps += P.desc[v].W[0] * P.desc[v].prep_tiles_x;
                 ++v;
             }
             R.v1 = v;
             R.ws1 = ws;
             R.fs1 = fs;
+            R.ps1 = ps;
 #ifndef N4_EMULATE
             R.stream = g == 0 ? stream : (void*)h->gstream[g];
 #else
@@ -770,15 +770,34 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
 #endif
         }
     }
+    // Every volume runs its own level/iteration state machine on the device;
+    // the host enqueues identical steps (hist -> wiener -> fit -> eval) per
+    // group, round-robin.  A group whose pipeline is full is skipped until
+    // its progress allows more, the host spins only when no group can
+    // advance, and a group ends once all its volumes are finished.  Every
+    // volume needs at most sum(max_iter + 1) + 1 steps.
+    int max_steps = 1;
+    for (int l = 0; l < P.nlevels; ++l) max_steps += P.max_iter[l] + 1;
+    size_t lds_fit_all = 0, lds_eval_all = 0;
+    for (int l = 0; l < P.nlevels; ++l) {
+        lds_fit_all = std::max(lds_fit_all, P.lds_fit_max[l]);
+        lds_eval_all = std::max(lds_eval_all, P.lds_eval_max[l]);
+    }
+    const int mt = P.mt[P.nlevels - 1], nt = P.nt[P.nlevels - 1];
 #ifndef N4_EMULATE
     if (G > 1) {
         N4_CHECK_HIP(hipEventRecord(h->fork_ev, (hipStream_t)stream));
         for (int g = 1; g < G; ++g) N4_CHECK_HIP(hipStreamWaitEvent(h->gstream[g], h->fork_ev, 0));
     }
 #endif
-    // round-robin over groups, one iteration (or level transition) at a time;
-    // a group whose pipeline is full is skipped until its progress allows
-    // more, and the host spins only when no group can advance
+    // each group prepares its own volumes on its own stream, so the
+    // full-resolution pass of one group overlaps the steps of the others
+    for (int g = 0; g < G; ++g) {
+        GroupRun& R = gr[g];
+        void* gs = R.stream;
+        GLAUNCH(N4_K_PREP, launch_prep(dd, dev_pslab(h, P) + R.ps0, R.ps1 - R.ps0, 0, rp, gs));
+        GLAUNCH(N4_K_LEVEL, launch_level(dd + R.v0, R.v1 - R.v0, 0, rp, gs));
+    }
     int remaining = G;
     uint64_t idle_rounds = 0;
     while (remaining > 0) {
@@ -787,42 +806,27 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             GroupRun& R = gr[g];
             if (R.finished) continue;
             void* gs = R.stream;
-            bool level_end = R.it >= P.max_iter[R.level];
-            if (R.started && R.it > 0 && !level_end) {
+            bool end = R.it >= max_steps;
+            if (R.it > 0 && !end) {
                 bool ready = false, all_done = false;
-                s = check_progress(h, R.v0, R.v1, R.level, R.it, gs, (idle_rounds & 63) == 63, &ready, &all_done);
+                s = check_progress(h, R.v0, R.v1, 0, R.it, gs, (idle_rounds & 63) == 63, &ready, &all_done);
                 if (s != N4_OK) return s;
                 if (!ready) continue;
-                level_end = all_done;
+                end = all_done;
             }
+            advanced = true;
             const int nvg = R.v1 - R.v0, nws = R.ws1 - R.ws0;
             const VolDesc* gv = dd + R.v0;
             const Slab* gsl = wsl + R.ws0;
-            const int l = R.level;
-            if (!R.started) {
-                GLAUNCH(N4_K_LEVEL, launch_level(gv, nvg, l, R.prev_iters, rp, gs));
-                GLAUNCH(N4_K_FIT_DEN, launch_fit(dd, gsl, nws, l, 0, kFitDen, rp, P.lds_fit_max[l], P.mt[l], P.nt[l], gs));
-                GLAUNCH(N4_K_LATTICE, launch_lattice(gv, nvg, P.cx_max[l], l, kFitDen, rp, gs));
-                R.started = true;
-            }
-            advanced = true;
-            if (!level_end) {
-                const int it = R.it;
-                GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, l, it, rp, nullptr, P.lds_hist, gs));
+            if (!end) {
+                GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, R.it, rp, nullptr, P.lds_hist, gs));
                 GLAUNCH(N4_K_WIENER, launch_wiener(gv, nvg, rp, P.lds_wiener, gs));
-                GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, l, it, kFitNum, rp, P.lds_fit_max[l], P.mt[l], P.nt[l], gs));
-                GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, l, it, rp, P.lds_eval_max[l], P.mt[l], P.nt[l], gs));
-                R.it = it + 1;
+                GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, 0, 0, kFitStep, rp, lds_fit_all, mt, nt, gs));
+                GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, rp, lds_eval_all, mt, nt, gs));
+                ++R.it;
                 continue;
             }
-            R.prev_iters = R.it;
-            R.it = 0;
-            R.started = false;
-            R.level = l + 1;
-            if (R.level < P.nlevels) continue;
-            GLAUNCH(N4_K_LEVEL, launch_level(gv, nvg, P.nlevels, R.prev_iters, rp, gs));
-            GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon,
-                                                 P.mt[P.nlevels - 1], P.nt[P.nlevels - 1], gs));
+            GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon, gs));
             for (int v = R.v0; v < R.v1; ++v)
                 if (in[v].out_lattice) {
                     const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
@@ -1074,8 +1078,8 @@ static n4_status flat_hist(n4_handle* h, const Plan& P, const float* d_x, RunPar
     VolDesc* dd = dev_desc(h, P);
     Slab* sl = dev_wslab(h, P);
     LAUNCH(launch_flat(dd, sl, P.n_wslab, kFlatMinMax, d_x, nullptr, nullptr, rp, nullptr));
-    LAUNCH(launch_level(dd, 1, 0, 0, rp, nullptr));
-    LAUNCH(launch_hist(dd, sl, P.n_wslab, 0, 0, rp, d_bin_index, P.lds_hist, nullptr));
+    LAUNCH(launch_level(dd, 1, 0, rp, nullptr));
+    LAUNCH(launch_hist(dd, sl, P.n_wslab, -1, rp, d_bin_index, P.lds_hist, nullptr));
     return N4_OK;
 }
 
@@ -1156,7 +1160,7 @@ n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_
     H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
     VolDesc* dd = dev_desc(h, P);
     Slab* sl = dev_wslab(h, P);
-    LAUNCH(launch_level(dd, 1, 0, 0, P.rp, nullptr));
+    LAUNCH(launch_level(dd, 1, 0, P.rp, nullptr));
     LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, 0, kFitDen, P.rp, P.lds_fit_max[0], P.mt[0], P.nt[0], nullptr));
     LAUNCH(launch_lattice(dd, 1, P.cx_max[0], 0, kFitDen, P.rp, nullptr));
     LAUNCH(launch_fit(dd, sl, P.n_wslab, 0, 0, kFitField, P.rp, P.lds_fit_max[0], P.mt[0], P.nt[0], nullptr));
@@ -1188,7 +1192,7 @@ n4_status n4_evaluate_bspline3d(n4_handle* h, const float* d_lattice, const uint
     D.latf = const_cast<float*>(d_lattice);
     memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
     H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
-    LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, P.mt[0], P.nt[0], nullptr));
+    LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, nullptr));
     SYNC(nullptr);
     return N4_OK;
 }
@@ -1244,7 +1248,7 @@ n4_status n4_upsample_lattice3d(n4_handle* h, const float* d_lattice, const uint
     SYNC(nullptr);
     for (size_t i = 0; i < L0; ++i) ld[i] = lf[i];
     H2D(D.lat[0], ld.data(), L0 * 8, nullptr);
-    LAUNCH(launch_level(dev_desc(h, P), 1, 1, 0, P.rp, nullptr));
+    LAUNCH(launch_level(dev_desc(h, P), 1, 1, P.rp, nullptr));
     D2D(d_out, D.latf, L1 * 4, nullptr);
     SYNC(nullptr);
     return N4_OK;

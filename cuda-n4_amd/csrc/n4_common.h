@@ -41,13 +41,26 @@ struct LevelDesc {
     uint64_t mCz;            // magic divisor for C[2]
 };
 
+// Each volume runs its own level/iteration state machine inside the same
+// launches ("steps": hist -> wiener -> fit -> eval).  At the start of a step
+// the histogram kernel turns the state (mode, lvl, it) into the step's kind;
+// the Wiener kernel (one block per volume) advances the state for the next
+// step.  Kinds:
+//   ITER(lvl, it)   histogram, Wiener step, fit numerator, lattice update
+//   TRANS(lvl)      enter level lvl: refine the lattice (lvl > 0), fit the
+//                   level's denominator
+//   FINAL           all levels done: final FP32 lattice
+//   SKIP            finished volume
+enum StateMode : int32_t { kModeTrans = 0, kModeIter = 1, kModeFinished = 2 };
+enum StepKind : int32_t { kStepSkip = 0, kStepIter = 1, kStepTrans = 2, kStepFinal = 3 };
+
 struct VolState {
     double bin_min, bin_max, slope;   // histogram range of logU over the mask
     double cv;                        // last convergence measurement
     int32_t degenerate;               // empty mask or max == min: no sharpening
-    int32_t done;                     // current level finished
-    int32_t iter;                     // iterations executed at current level
-    int32_t level;
+    int32_t done;                     // all levels finished
+    int32_t mode, lvl, it;            // state at the start of the next step
+    int32_t s_kind, s_lvl, s_it;      // the current step (written by the histogram kernel)
     uint64_t n_mask_full, n_mask_work;
     int32_t iters[kMaxLevels];
     double lastcv[kMaxLevels];
@@ -126,8 +139,8 @@ struct Slab {
 
 // Progress word of one volume, written by the device into pinned host
 // memory (one 64-bit store, so the fields are always read consistently):
-// bits 40.. level + 1, bits 1..39 histogram passes run at that level
-// (pass it+1 runs only after iteration it-1 has finished), bit 0 done.
+// bits 40.. level + 1, bits 1..39 histogram passes run (pass k runs only
+// after step k-2 has finished), bit 0 done.
 constexpr uint64_t progress_word(int level, int passes, int done) {
     return ((uint64_t)(level + 1) << 40) | ((uint64_t)passes << 1) | (uint64_t)(done ? 1 : 0);
 }
@@ -139,7 +152,12 @@ constexpr int progress_done(uint64_t w) { return (int)(w & 1); }
 // m = floor(2^40 / d) + 1 (computed on the host).
 constexpr int kDivShift = 40;
 
-enum FitMode : int32_t { kFitNum = 0, kFitDen = 1, kFitField = 2 };
+// Fit modes.  NUM and DEN_SCATTER add the slab's contraction, weighted by
+// the x factor, into the fixed-point numerator buffers (the correction
+// loop); DEN and FIELD write per-slab partials reduced by the lattice
+// kernel (stage calls).  STEP takes the mode, level and iteration from the
+// volume's current step (ITER -> NUM, TRANS -> DEN_SCATTER, else nothing).
+enum FitMode : int32_t { kFitNum = 0, kFitDen = 1, kFitField = 2, kFitDenScatter = 3, kFitStep = 4 };
 
 // Fixed-point scale of the lattice numerator accumulated by the fit's
 // integer atomics (order-independent, so bitwise reproducible): 2^-30

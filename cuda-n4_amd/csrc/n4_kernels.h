@@ -10,7 +10,8 @@
 // written as per-slab partials and reduced in a fixed order by a later
 // kernel, so results are bitwise reproducible.
 //
-// One fitting iteration = hist -> wiener -> fit -> lattice -> eval.
+// One correction step = hist -> wiener -> fit -> eval, each volume at its
+// own level and iteration (see VolState).
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -24,12 +25,11 @@ size_t lds_prep();
 size_t lds_level();
 size_t lds_hist(int nbins);
 size_t lds_wiener(int P);
-// prec (RunParams::prec) selects the FP32 or the MFMA kernels; recon_rows =
-// full-resolution rows per recon workgroup.
+// prec (RunParams::prec) selects the FP32 or the MFMA kernels.
 size_t lds_fit(int nbins, const VolDesc& v, int level, int prec);
 size_t lds_lattice();
 size_t lds_eval(const VolDesc& v, int level, int prec);
-size_t lds_recon(const VolDesc& v, int recon_rows, int prec);
+size_t lds_recon(const VolDesc& v);
 // MFMA tiles (16 control points each) for c control points along an axis.
 int mfma_tiles(int c);
 size_t lds_flat(int nbins);
@@ -37,11 +37,14 @@ size_t lds_flat(int nbins);
 // `stream` is a hipStream_t (void* keeps this header HIP-free).  Return 0
 // on success, else a hipError_t value.
 int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink_hint, RunParams rp, void* stream);
-// level in [0, nlevels]: commits the previous level's last iteration
-// (prev_iters enqueued) and, if level < nlevels, sets the level up.
-int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunParams rp, void* stream);
-int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
-                int32_t* dbg_idx, size_t lds, void* stream);
+// level 0: start of a correction (counts, cleared accumulators, state
+// TRANS(0)); level > 0: refinement of level-1's lattice (stage call).
+int launch_level(const VolDesc* vols, int nvol, int level, RunParams rp, void* stream);
+// One correction step is hist -> wiener -> fit(kFitStep) -> eval; each
+// volume's step kind comes from its own state.  step: index of the step
+// (published as progress); step < 0 (stage calls) forces a histogram.
+int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int step, RunParams rp, int32_t* dbg_idx,
+                size_t lds, void* stream);
 int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void* stream);
 // mt, nt: MFMA tiles along y and z of the level's lattice (used when
 // rp.prec != 0; the MFMA kernels exist for 1..4 tiles per axis).
@@ -49,10 +52,11 @@ int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void*
 int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, int mode, RunParams rp,
                size_t lds, int mt, int nt, void* stream);
 int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mode, RunParams rp, void* stream);
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp, size_t lds,
-                int mt, int nt, void* stream);
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, int mt,
-                 int nt, void* stream);
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, size_t lds, int mt, int nt,
+                void* stream);
+// The final full-resolution reconstruction: FP32 in every precision mode.
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
+                 void* stream);
 
 // Flat (1-D) stage helpers used by the stage entry points.  A flat array of
 // n voxels is cut into slabs of kChunkVox voxels; each slab writes

@@ -120,23 +120,52 @@ HD double red_apply(int op, double a, double b) {
     return op == 0 ? a + b : (op == 1 ? fmin(a, b) : fmax(a, b));
 }
 
-// Tree reduction of four kBlock-long LDS arrays (op: 0 sum, 1 min, 2 max);
-// results in red[0], red[kBlock], red[2*kBlock], red[3*kBlock].  The order
-// of operations is fixed, so every block reducing the same data gets the
-// same bits.
+// Reduction of four kBlock-long LDS arrays (op: 0 sum, 1 min, 2 max);
+// results in red[0], red[kBlock], red[2*kBlock], red[3*kBlock].  Each
+// wave64 reduces its lanes with an xor butterfly (every lane ends with the
+// same value: the ops are commutative), then the wave results combine in
+// wave order.  The order of operations is fixed, so every block reducing
+// the same data gets the same bits; the host emulation performs the same
+// operations in the same order.
 HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
     BLOCK_SYNC();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        BLOCK_FOR(t)
-        if (t < s) {
-            red[t] = red_apply(o0, red[t], red[t + s]);
-            red[kBlock + t] = red_apply(o1, red[kBlock + t], red[kBlock + t + s]);
-            red[2 * kBlock + t] = red_apply(o2, red[2 * kBlock + t], red[2 * kBlock + t + s]);
-            red[3 * kBlock + t] = red_apply(o3, red[3 * kBlock + t], red[3 * kBlock + t + s]);
+    const int ops[4] = {o0, o1, o2, o3};
+    constexpr int kWaves = kBlock / 64;
+#ifdef N4_DEVICE_PASS
+    const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+    double v[4];
+    UNROLL for (int k = 0; k < 4; ++k) v[k] = red[k * kBlock + t];
+    UNROLL for (int sh = 32; sh > 0; sh >>= 1)
+        UNROLL for (int k = 0; k < 4; ++k) v[k] = red_apply(ops[k], v[k], __shfl_xor(v[k], sh));
+    __syncthreads();
+    if (lane == 0)
+        UNROLL for (int k = 0; k < 4; ++k) red[k * kBlock + w] = v[k];
+    __syncthreads();
+    if (t == 0) {
+        UNROLL for (int k = 0; k < 4; ++k) {
+            double r = red[k * kBlock];
+            for (int i = 1; i < kWaves; ++i) r = red_apply(ops[k], r, red[k * kBlock + i]);
+            red[k * kBlock] = r;
         }
-        BLOCK_END
-        BLOCK_SYNC();
     }
+    __syncthreads();
+#else
+    for (int k = 0; k < 4; ++k) {
+        double wres[kWaves];
+        for (int w = 0; w < kWaves; ++w) {
+            double v[64], nv[64];
+            for (int l = 0; l < 64; ++l) v[l] = red[k * kBlock + 64 * w + l];
+            for (int sh = 32; sh > 0; sh >>= 1) {
+                for (int l = 0; l < 64; ++l) nv[l] = red_apply(ops[k], v[l], v[l ^ sh]);
+                for (int l = 0; l < 64; ++l) v[l] = nv[l];
+            }
+            wres[w] = v[0];
+        }
+        double r = wres[0];
+        for (int i = 1; i < kWaves; ++i) r = red_apply(ops[k], r, wres[i]);
+        red[k * kBlock] = r;
+    }
+#endif
 }
 
 // Reduces n partial quadruples part[i] = (s0, s1, min, max).
@@ -306,141 +335,128 @@ HD double refine_w(int odd, int a) {
     return odd ? (a == 0 ? 0.0 : 0.5) : (a == 1 ? 0.75 : 0.125);
 }
 
+// Lattice of `level` (> 0) as the dyadic refinement of level-1's FP64
+// lattice (N4.py:132-165, cudaN4.cu:257-339) with per-axis clip bounds;
+// also its FP32 copy.  Runs inside a thread phase (thread t of kBlock).
+HD void refine_lattice(const VolDesc& V, int level, int t) {
+    const LevelDesc& LD = V.lev[level];
+    const LevelDesc& PD = V.lev[level - 1];
+    const double* src = V.lat[(level - 1) & 1];
+    double* dst = V.lat[level & 1];
+    for (int l = t; l < LD.L; l += kBlock) {
+        int c[3];
+        c[0] = l / (LD.C[1] * LD.C[2]);
+        c[1] = (l / LD.C[2]) % LD.C[1];
+        c[2] = l % LD.C[2];
+        int par[3], odd[3];
+        for (int d = 0; d < 3; ++d) {
+            const int ic = c[d] - 1;  // child index in [-1, 2n+1]
+            const int i = (ic >= 0) ? ic / 2 : -((-ic + 1) / 2);
+            par[d] = i;
+            odd[d] = ic - 2 * i;
+        }
+        double acc = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            int X = par[0] - 1 + a;
+            const int nx = PD.C[0] - 3;
+            X = X < -1 ? -1 : (X > nx + 1 ? nx + 1 : X);
+            for (int b = 0; b < 3; ++b) {
+                int Y = par[1] - 1 + b;
+                const int ny = PD.C[1] - 3;
+                Y = Y < -1 ? -1 : (Y > ny + 1 ? ny + 1 : Y);
+                for (int e = 0; e < 3; ++e) {
+                    int Z = par[2] - 1 + e;
+                    const int nz = PD.C[2] - 3;
+                    Z = Z < -1 ? -1 : (Z > nz + 1 ? nz + 1 : Z);
+                    const double wgt = refine_w(odd[0], a) * refine_w(odd[1], b) * refine_w(odd[2], e);
+                    acc += wgt * src[((size_t)(X + 1) * PD.C[1] + (Y + 1)) * PD.C[2] + (Z + 1)];
+                }
+            }
+        }
+        dst[l] = acc;
+        V.latf[l] = (float)acc;
+    }
+}
+
 // One block per volume.
-//  1. level > 0: commits the previous level's last iteration (prev_iters
-//     enqueued by the host) unless the volume already stopped.
-//  2. level == 0: reduces the prep partials to the mask counts, zeroes the
-//     lattice and the histogram accumulator.
-//     0 < level < nlevels: dyadic refinement of the previous level's FP64
-//     lattice (N4.py:132-165, cudaN4.cu:257-339), per-axis clip bounds.
-//  level == nlevels only commits.
-HD void body_level(const VolDesc* vols, int level, int prev_iters, RunParams rp, int bid, unsigned char* smem) {
+//  level == 0: start of a correction (or of a stage call): reduces the prep
+//    partials to the mask counts, seeds the slab partials with the logU
+//    range, clears lattice and accumulators and puts the volume in state
+//    TRANS(0).
+//  level > 0: refines the level-1 lattice (lattice refinement stage call).
+HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsigned char* smem) {
     const VolDesc& V = vols[bid];
     VolState* st = V.st;
-    Carve cv{smem, 0};
-    double* red = cv.take<double>(4 * kBlock);
-    if (level == 0)
-        reduce_partials(V.ppart, V.W[0] * V.prep_tiles_x, red);
-    else
-        reduce_partials(V.spart, V.W[0], red);
-    if (level == 0) {
-        // seed the slab partials with the initial logU range for iteration 0
+    if (level > 0) {
         BLOCK_FOR(t)
-        for (int xx = t; xx < V.W[0]; xx += kBlock) {
-            double* sp = V.spart + (size_t)xx * 4;
-            sp[0] = 0.0;
-            sp[1] = 0.0;
-            sp[2] = xx == 0 ? red[2 * kBlock] : INFINITY;
-            sp[3] = xx == 0 ? red[3 * kBlock] : -INFINITY;
-        }
-        BLOCK_END
-    }
-    BLOCK_FOR(t)
-    if (t == 0) {
-        if (level == 0) {
-            st->n_mask_full = (uint64_t)red[0];
-            st->n_mask_work = (uint64_t)red[kBlock];
-        } else if (!st->done && prev_iters > 0) {
-            const double cvv = cv_from_sums(red[0], red[kBlock], (double)st->n_mask_work);
-            st->iters[level - 1] = prev_iters;
-            st->lastcv[level - 1] = cvv;
-            st->cv = cvv;
-        }
-    }
-    BLOCK_END
-    if (level >= V.nlevels) {
-        // final lattice in FP32 for the reconstruction and the lattice output
-        const LevelDesc& FD = V.lev[V.nlevels - 1];
-        const double* fin = V.lat[(V.nlevels - 1) & 1];
-        BLOCK_FOR(t)
-        for (int l = t; l < FD.L; l += kBlock) V.latf[l] = (float)fin[l];
+        refine_lattice(V, level, t);
         BLOCK_END
         return;
     }
-    const LevelDesc& LD = V.lev[level];
-    double* dst = V.lat[level & 1];
-    if (level == 0) {
-        BLOCK_FOR(t)
-        for (int l = t; l < LD.L; l += kBlock) {
-            dst[l] = 0.0;
-            V.latf[l] = 0.0f;
-            V.lresf[l] = 0.0f;
-            V.numfx[0][l] = 0;
-            V.numfx[1][l] = 0;
-        }
-        for (int i = t; i < rp.nbins; i += kBlock) V.hist[i] = 0;
-        BLOCK_END
-    } else {
-        BLOCK_FOR(t)
-        for (int l = t; l < LD.L; l += kBlock) {
-            V.numfx[0][l] = 0;
-            V.numfx[1][l] = 0;
-        }
-        BLOCK_END
-        const LevelDesc& PD = V.lev[level - 1];
-        const double* src = V.lat[(level - 1) & 1];
-        BLOCK_FOR(t)
-        for (int l = t; l < LD.L; l += kBlock) {
-            int c[3];
-            c[0] = l / (LD.C[1] * LD.C[2]);
-            c[1] = (l / LD.C[2]) % LD.C[1];
-            c[2] = l % LD.C[2];
-            int par[3], odd[3];
-            for (int d = 0; d < 3; ++d) {
-                const int ic = c[d] - 1;  // child index in [-1, 2n+1]
-                const int i = (ic >= 0) ? ic / 2 : -((-ic + 1) / 2);
-                par[d] = i;
-                odd[d] = ic - 2 * i;
-            }
-            double acc = 0.0;
-            for (int a = 0; a < 3; ++a) {
-                int X = par[0] - 1 + a;
-                const int nx = PD.C[0] - 3;
-                X = X < -1 ? -1 : (X > nx + 1 ? nx + 1 : X);
-                for (int b = 0; b < 3; ++b) {
-                    int Y = par[1] - 1 + b;
-                    const int ny = PD.C[1] - 3;
-                    Y = Y < -1 ? -1 : (Y > ny + 1 ? ny + 1 : Y);
-                    for (int e = 0; e < 3; ++e) {
-                        int Z = par[2] - 1 + e;
-                        const int nz = PD.C[2] - 3;
-                        Z = Z < -1 ? -1 : (Z > nz + 1 ? nz + 1 : Z);
-                        const double wgt = refine_w(odd[0], a) * refine_w(odd[1], b) * refine_w(odd[2], e);
-                        acc += wgt * src[((size_t)(X + 1) * PD.C[1] + (Y + 1)) * PD.C[2] + (Z + 1)];
-                    }
-                }
-            }
-            dst[l] = acc;
-            V.latf[l] = (float)acc;
-            V.lresf[l] = 0.0f;
-        }
-        BLOCK_END
-    }
+    Carve cv{smem, 0};
+    double* red = cv.take<double>(4 * kBlock);
+    reduce_partials(V.ppart, V.W[0] * V.prep_tiles_x, red);
+    const LevelDesc& LD = V.lev[0];
+    int Lmax = 0;
+    for (int l = 0; l < V.nlevels; ++l) Lmax = V.lev[l].L > Lmax ? V.lev[l].L : Lmax;
     BLOCK_FOR(t)
+    for (int xx = t; xx < V.W[0]; xx += kBlock) {
+        double* sp = V.spart + (size_t)xx * 4;
+        sp[0] = 0.0;
+        sp[1] = 0.0;
+        sp[2] = xx == 0 ? red[2 * kBlock] : INFINITY;
+        sp[3] = xx == 0 ? red[3 * kBlock] : -INFINITY;
+    }
+    for (int l = t; l < LD.L; l += kBlock) {
+        V.lat[0][l] = 0.0;
+        V.latf[l] = 0.0f;
+        V.lresf[l] = 0.0f;
+    }
+    for (int l = t; l < Lmax; l += kBlock) {
+        V.numfx[0][l] = 0;
+        V.numfx[1][l] = 0;
+    }
+    for (int i = t; i < rp.nbins; i += kBlock) V.hist[i] = 0;
     if (t == 0) {
-        st->level = level;
-        st->iter = 0;
+        st->n_mask_full = (uint64_t)red[0];
+        st->n_mask_work = (uint64_t)red[kBlock];
+        st->mode = kModeTrans;
+        st->lvl = 0;
+        st->it = 0;
+        st->s_kind = kStepSkip;
+        st->s_lvl = 0;
+        st->s_it = 0;
+        st->done = 0;
         st->cv = 1e10;
-        st->done = (V.max_iter[level] <= 0) ? 1 : 0;
-        publish_progress(V, level, 0, st->done);
-        st->iters[level] = 0;
-        st->lastcv[level] = 1e10;
+        for (int l = 0; l < kMaxLevels; ++l) {
+            st->iters[l] = 0;
+            st->lastcv[l] = 1e10;
+        }
+        publish_progress(V, 0, 0, 0);
     }
     BLOCK_END
 }
 
 // ------------------------------------------------------------- histogram --
 // Prologue (every block of the volume computes identical bits): reduce the
-// eval partials of the previous iteration -> logU min/max (N4.py:14) and,
-// for it > 0, the convergence measurement of iteration it-1 and the stop
-// decision (N4.py:301-302).  The block of slab 0 commits them to the volume
-// state.  Then the fractional (linear-split) histogram of logU over the
+// eval partials of the previous step -> logU min/max (N4.py:14) and the
+// convergence measurement of the previous iteration, then the step's kind
+// from the volume's state:
+//   TRANS(l)            -> TRANS(l)
+//   ITER(l, it), it > 0 and CV <= threshold (N4.py:301-302), or it at the
+//   level's maximum     -> the level ends after `it` iterations: TRANS(l+1),
+//                          or FINAL after the last level
+//   ITER(l, it) else    -> ITER(l, it)
+//   finished            -> SKIP
+// The block of slab 0 commits the level record and the step and publishes
+// the volume's progress.  step < 0 (stage calls) forces ITER(0, 0).  Then,
+// for ITER steps, the fractional (linear-split) histogram of logU over the
 // mask, N4.py:19-29 (cudaN4.cu:153-220): bin index in FP64 exactly as N4.py
 // computes it; contributions accumulate as 2^-32 fixed point in LDS-private
 // u64 bins, then in the volume's global u64 bins (integer adds:
 // order-independent, so bitwise deterministic).
-HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, RunParams rp, int32_t* dbg_idx,
-                  int bid, unsigned char* smem) {
+HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp, int32_t* dbg_idx, int bid,
+                  unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     VolState* st = V.st;
@@ -448,36 +464,59 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int level, int it, Run
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
     unsigned long long* h = cv.take<unsigned long long>(nb);
+    const int mode = step < 0 ? kModeIter : st->mode;
+    const int lvl = step < 0 ? 0 : st->lvl, it = step < 0 ? 0 : st->it;
+    if (mode == kModeFinished) {
+        if (sl.x == 0) {
+            BLOCK_FOR(t)
+            if (t == 0) st->s_kind = kStepSkip;
+            BLOCK_END
+        }
+        return;
+    }
     reduce_partials(V.spart, V.W[0], red);
     const double n = (double)st->n_mask_work;
     const double mn = red[2 * kBlock], mx = red[3 * kBlock];
     double cvv = 0.0;
-    bool stop = false;
-    if (it > 0) {
-        cvv = cv_from_sums(red[0], red[kBlock], n);
-        stop = !rp.fixed_iters && !(cvv > rp.thr);
+    int kind = kStepTrans, s_lvl = lvl;
+    bool level_end = false;
+    if (mode == kModeIter) {
+        if (it > 0) cvv = cv_from_sums(red[0], red[kBlock], n);
+        const bool stop = it > 0 && !rp.fixed_iters && !(cvv > rp.thr);
+        level_end = stop || it >= V.max_iter[lvl];
+        if (!level_end) {
+            kind = kStepIter;
+        } else if (lvl + 1 < V.nlevels) {
+            kind = kStepTrans;
+            s_lvl = lvl + 1;
+        } else {
+            kind = kStepFinal;
+        }
     }
     const double slope = (mx - mn) / (double)(nb - 1);
     const bool degenerate = !(n > 0.0) || !(slope > 0.0);
     if (sl.x == 0) {
         BLOCK_FOR(t)
-        if (t == 0 && !st->done) {
-            if (it > 0) {
-                st->iters[level] = it;
-                st->lastcv[level] = cvv;
-                st->cv = cvv;
-                if (stop) st->done = 1;
+        if (t == 0) {
+            if (level_end) {
+                st->iters[lvl] = it;
+                if (it > 0) {
+                    st->lastcv[lvl] = cvv;
+                    st->cv = cvv;
+                }
             }
-            publish_progress(V, level, it + 1, stop ? 1 : 0);
-            st->iter = it;
+            st->s_kind = kind;
+            st->s_lvl = s_lvl;
+            st->s_it = it;
             st->bin_min = mn;
             st->bin_max = mx;
             st->slope = slope;
             st->degenerate = degenerate ? 1 : 0;
+            if (step >= 0) publish_progress(V, 0, step + 1, kind == kStepFinal ? 1 : 0);
         }
         BLOCK_END
     }
-    if (stop || degenerate) return;
+    if (kind != kStepIter || degenerate) return;
     const int YZ = V.W[1] * V.W[2];
     const size_t base = (size_t)sl.x * YZ;
     BLOCK_FOR(t)
@@ -616,14 +655,11 @@ HD D2* lds_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
     return a;
 }
 
-// One block per volume: Wiener deconvolution of the fractional histogram
-// and the E[u|v] table, N4.py:31-101 (cudaN4.cu:733-820 did this on the
-// host with FFTW).  Two real transforms share one complex FFT (V in the real
-// part, F in the imaginary).  Reads and clears the volume's histogram.
-HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* smem) {
-    const VolDesc& V = vols[bid];
-    const VolState* st = V.st;
-    if (st->done || st->degenerate) return;
+// Wiener deconvolution of the fractional histogram and the E[u|v] table,
+// N4.py:31-101 (cudaN4.cu:733-820 did this on the host with FFTW).  Two real
+// transforms share one complex FFT (V in the real part, F in the
+// imaginary).  Reads and clears the volume's histogram.
+HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigned char* smem) {
     const int P = rp.P, half = P / 2, off = rp.offset, nb = rp.nbins;
     Carve cv{smem, 0};
     D2* z = cv.take<D2>(P);
@@ -638,15 +674,16 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
     for (int j = t; j < P; j += kBlock) {
         const int jj = (j <= half) ? j : P - j;
         const double Fj = (j == 0) ? scale : scale * exp((double)(-(int64_t)jj * jj) * expF);
-        const double Vj = (j >= off && j < off + nb) ? (double)V.hist[j - off] * 0x1p-32 : 0.0;
+        double Vj = 0.0;
+        if (j >= off && j < off + nb) {
+            Vj = (double)V.hist[j - off] * 0x1p-32;
+            V.hist[j - off] = 0;
+        }
         z[j] = D2{Vj, Fj};
         tw[j] = rp.twiddle[j];
     }
     BLOCK_END
     BLOCK_SYNC();
-    BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock) V.hist[i] = 0;
-    BLOCK_END
     D2* Z = lds_fft(z, y, tw, P, false);
     D2* U = (Z == z) ? y : z;
     // Split Vf and Ff, Gf = conj(Ff)/(conj(Ff)Ff + noise), Uf = Vf*Re(Gf)
@@ -689,17 +726,66 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
     BLOCK_END
 }
 
+// One block per volume: the step's per-volume work and the state for the
+// next step.
+//   ITER   Wiener step (unless degenerate); next: ITER(l, it+1)
+//   TRANS  refine the lattice into level l (l > 0), clear the numerator
+//          buffers; next: ITER(l, 0)
+//   FINAL  FP32 copy of the final lattice; next: finished
+HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* smem) {
+    const VolDesc& V = vols[bid];
+    VolState* st = V.st;
+    const int kind = st->s_kind;
+    if (kind == kStepIter) {
+        if (!st->degenerate) wiener_table(V, st, rp, smem);
+        BLOCK_FOR(t)
+        if (t == 0) {
+            st->mode = kModeIter;
+            st->it = st->s_it + 1;
+        }
+        BLOCK_END
+    } else if (kind == kStepTrans) {
+        const int l = st->s_lvl;
+        BLOCK_FOR(t)
+        if (l > 0) refine_lattice(V, l, t);
+        for (int i = t; i < V.lev[l].L; i += kBlock) {
+            V.numfx[0][i] = 0;
+            V.numfx[1][i] = 0;
+        }
+        if (t == 0) {
+            st->mode = kModeIter;
+            st->lvl = l;
+            st->it = 0;
+            st->cv = 1e10;
+        }
+        BLOCK_END
+    } else if (kind == kStepFinal) {
+        const LevelDesc& FD = V.lev[V.nlevels - 1];
+        const double* fin = V.lat[(V.nlevels - 1) & 1];
+        BLOCK_FOR(t)
+        for (int l = t; l < FD.L; l += kBlock) V.latf[l] = (float)fin[l];
+        if (t == 0) {
+            st->mode = kModeFinished;
+            st->done = 1;
+        }
+        BLOCK_END
+    }
+}
+
 // ---------------------------------------------------------- lattice update --
 // The fit of slab x adds its (y, z) contraction T2, weighted by the x
 // factor Px[x, cx], to the numerator of the 4 control planes cx = b(x) ..
 // b(x)+3 it touches, as fixed-point integer atomics into numfx[it & 1]
 // (integer sums do not depend on order, so the result is bitwise
 // reproducible): num[cx,p] = sum_x Px[x,cx] T2_x[p]  (N4.py:186-237).
-HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry& ex, int it, int pr, float t2) {
+// The denominator of a level (mask contracted with the w^2 factors) is
+// accumulated the same way, into buffer 0, in the level's first step.
+HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry& ex, int buf, bool q, int pr,
+                          float t2) {
     const int CyCz = LD.C[1] * LD.C[2];
-    uint64_t* acc = V.numfx[it & 1];
+    uint64_t* acc = V.numfx[buf];
     UNROLL for (int k = 0; k < 4; ++k) {
-        const double c = (double)ex.p[k] * (double)t2;
+        const double c = (double)(q ? ex.q[k] : ex.p[k]) * (double)t2;
         const long long q = (long long)rint(c * kNumScale);
         global_add_u64(acc + (size_t)(ex.b + k) * CyCz + pr, (unsigned long long)q);
     }
@@ -732,6 +818,24 @@ HD void lattice_planes(const VolDesc& V, const LevelDesc& LD, double* lat, int x
     }
 }
 
+// First step of a level: the owner block of each of slab x's control planes
+// moves the fixed-point denominator into den (FP64) and clears buffer 0 for
+// the level's first numerator.  Runs inside a thread phase.
+HD void finalize_denominator(const VolDesc& V, int level, int x, int t) {
+    const LevelDesc& LD = V.lev[level];
+    const AxisEntry& ex = LD.ax[0][x];
+    const int CyCz = LD.C[1] * LD.C[2];
+    for (int k = 0; k < 4; ++k) {
+        const int cx = ex.b + k;
+        if (x != LD.sup[0][2 * cx]) continue;
+        for (int pr = t; pr < CyCz; pr += kBlock) {
+            const size_t l = (size_t)cx * CyCz + pr;
+            V.den[l] = (double)(long long)V.numfx[0][l] / kNumScale;
+            V.numfx[0][l] = 0;
+        }
+    }
+}
+
 // ------------------------------------------------------------------- fit --
 // Remap (N4.py:103-115) + residual (N4.py:313) + the (y, z) part of the
 // separable B-spline fit (N4.py:186-237 as mode-n contractions, exact
@@ -753,6 +857,24 @@ struct FitLds {
     float* yw;
     int* yb;
 };
+
+// Resolves kFitStep from the volume's current step; false: nothing to fit.
+HD bool fit_step_mode(const VolState* st, int* mode, int* level, int* it) {
+    if (*mode != kFitStep) return true;
+    if (st->s_kind == kStepIter) {
+        *mode = kFitNum;
+        *level = st->s_lvl;
+        *it = st->s_it;
+        return true;
+    }
+    if (st->s_kind == kStepTrans) {
+        *mode = kFitDenScatter;
+        *level = st->s_lvl;
+        *it = 0;
+        return true;
+    }
+    return false;
+}
 
 // Contracted quantity g of one working voxel: the residual logU - E[u|v]
 // (N4.py:103-115 remap with the cudaN4.cu:242 clamp, then N4.py:313) for
@@ -793,7 +915,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     const VolState* st = V.st;
-    if (mode == kFitNum && st->done) return;
+    if (!fit_step_mode(st, &mode, &level, &it)) return;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
@@ -802,6 +924,9 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     FitLds S;
     carve_fit(smem, nb, Z, R, Cy, Cz, &S);
     const bool num = (mode == kFitNum);
+    const bool den = (mode == kFitDen || mode == kFitDenScatter);
+    const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
+    const int gmode = den ? (int)kFitDen : mode;
     const bool degenerate = num && st->degenerate;
     const double bmin = st->bin_min, slope = st->slope;
     const AxisEntry* axz = LD.ax[2];
@@ -812,7 +937,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         for (int i = t; i < nb; i += kBlock) S.E[i] = V.etab[i];
     for (int z = t; z < Z; z += kBlock) {
         const AxisEntry& e = axz[z];
-        for (int a = 0; a < 4; ++a) S.zw[4 * z + a] = (mode == kFitDen) ? e.q[a] : e.p[a];
+        for (int a = 0; a < 4; ++a) S.zw[4 * z + a] = den ? e.q[a] : e.p[a];
         S.zb[z] = e.b;
     }
     for (int i = t; i < 2 * Cz; i += kBlock) S.zs[i] = LD.sup[2][i];
@@ -826,7 +951,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         BLOCK_FOR(t)
         for (int r = t; r < ny; r += kBlock) {
             const AxisEntry& e = axy[y0 + r];
-            for (int a = 0; a < 4; ++a) S.yw[4 * r + a] = (mode == kFitDen) ? e.q[a] : e.p[a];
+            for (int a = 0; a < 4; ++a) S.yw[4 * r + a] = den ? e.q[a] : e.p[a];
             S.yb[r] = e.b;
         }
         for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
@@ -836,12 +961,12 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
                 const int v = v0 + u * kBlock + t;
                 const bool ok = v < ny * Z;
                 ms[u] = ok ? V.wmask[base + v] : 0;
-                xs[u] = (ok && mode != kFitDen) ? V.logU[base + v] : 0.0f;
+                xs[u] = (ok && !den) ? V.logU[base + v] : 0.0f;
             }
             UNROLL for (int u = 0; u < 4; ++u) {
                 const int v = v0 + u * kBlock + t;
                 if (v >= ny * Z) continue;
-                S.g[v] = fit_input(mode, ms[u], xs[u], degenerate, S.E, bmin, slope, nb);
+                S.g[v] = fit_input(gmode, ms[u], xs[u], degenerate, S.E, bmin, slope, nb);
             }
         }
         BLOCK_END
@@ -876,8 +1001,8 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     const AxisEntry ex = LD.ax[0][x];
     BLOCK_FOR(t)
     for (int pr = t; pr < Cy * Cz; pr += kBlock) {
-        if (num)
-            scatter_numerator(V, LD, ex, it, pr, S.T2[pr]);
+        if (scatter)
+            scatter_numerator(V, LD, ex, num ? (it & 1) : 0, den, pr, S.T2[pr]);
         else
             V.part[(size_t)x * V.part_stride + pr] = S.T2[pr];
     }
@@ -975,11 +1100,18 @@ HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds*
     return align16(c.off);
 }
 
-HD void body_eval(const VolDesc* vols, const Slab* slabs, int level, int it, RunParams rp, int bid,
-                  unsigned char* smem) {
+HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    if (V.st->done) return;
+    const int kind = V.st->s_kind;
+    if (kind == kStepTrans) {
+        BLOCK_FOR(t)
+        finalize_denominator(V, V.st->s_lvl, sl.x, t);
+        BLOCK_END
+        return;
+    }
+    if (kind != kStepIter) return;
+    const int level = V.st->s_lvl, it = V.st->s_it;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
@@ -1287,21 +1419,14 @@ size_t lds_eval(const VolDesc& v, int level, int prec) {
 #ifndef N4_EMULATE
     if (prec)
         return carve_eval_mfma(nullptr, v.W[1], v.W[2], L.C[1], L.C[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec,
-                               true, nullptr);
+                               nullptr);
 #endif
     (void)prec;
     return carve_eval(nullptr, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
 }
 
-size_t lds_recon(const VolDesc& v, int rows, int prec) {
+size_t lds_recon(const VolDesc& v) {
     const LevelDesc& L = v.lev[v.nlevels - 1];
-#ifndef N4_EMULATE
-    if (prec)
-        return carve_eval_mfma(nullptr, rows, v.F[2], L.C[1], L.C[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec,
-                               false, nullptr);
-#endif
-    (void)prec;
-    (void)rows;
     return carve_recon(nullptr, v.F[2], rows_per_chunk(v.F[1], v.F[2]), L.C[1], L.C[2], nullptr);
 }
 
@@ -1317,15 +1442,15 @@ __global__ __launch_bounds__(kBlock) void k_prep(const VolDesc* vols, const Slab
     body_prep(vols, slabs, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_level(const VolDesc* vols, int level, int prev_iters, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_level(const VolDesc* vols, int level, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_level(vols, level, prev_iters, rp, blockIdx.x, SMEM());
+    body_level(vols, level, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_hist(const VolDesc* vols, const Slab* slabs, int level, int it,
-                                                 RunParams rp, int32_t* dbg_idx) {
+__global__ __launch_bounds__(kBlock) void k_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp,
+                                                 int32_t* dbg_idx) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_hist(vols, slabs, level, it, rp, dbg_idx, blockIdx.x, SMEM());
+    body_hist(vols, slabs, step, rp, dbg_idx, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_wiener(const VolDesc* vols, RunParams rp) {
@@ -1344,10 +1469,9 @@ __global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* vols, int lev
     body_lattice(vols, level, mode, rp, blockIdx.x, blockIdx.y, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, int level, int it,
-                                                 RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_eval(vols, slabs, level, it, rp, blockIdx.x, SMEM());
+    body_eval(vols, slabs, rp, blockIdx.x, SMEM());
 }
 
 __global__ __launch_bounds__(kBlock) void k_recon(const VolDesc* vols, const Slab* slabs, int level,
@@ -1370,23 +1494,15 @@ __global__ __launch_bounds__(kBlock) void k_fit_mfma(const VolDesc* vols, const 
 }
 
 template <int PREC, int MT, int NT>
-__global__ __launch_bounds__(kBlock) void k_eval_mfma(const VolDesc* vols, const Slab* slabs, int level, int it,
-                                                      RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_eval_mfma<PREC, MT, NT>(vols, slabs, level, it, rp, blockIdx.x, SMEM());
-}
-
-template <int PREC, int MT, int NT>
-__global__ __launch_bounds__(kBlock) void k_recon_mfma(const VolDesc* vols, const Slab* slabs, int level,
-                                                       RunParams rp) {
-    HIP_DYNAMIC_SHARED(double, smem_);
-    body_recon_mfma<PREC, MT, NT>(vols, slabs, level, rp, blockIdx.x, SMEM());
+    body_eval_mfma<PREC, MT, NT>(vols, slabs, rp, blockIdx.x, SMEM());
 }
 
 // Launchers of every (precision, Cy tiles, Cz tiles) instance, indexed
 // [prec-1][MT-1][NT-1].
 typedef void (*FitLaunch)(int, size_t, hipStream_t, const VolDesc*, const Slab*, int, int, int, RunParams);
-typedef void (*EvalLaunch)(int, size_t, hipStream_t, const VolDesc*, const Slab*, int, int, RunParams);
+typedef void (*EvalLaunch)(int, size_t, hipStream_t, const VolDesc*, const Slab*, int, RunParams);
 
 template <int PREC, int MT, int NT>
 static void fit_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int level, int it,
@@ -1394,23 +1510,16 @@ static void fit_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, 
     hipLaunchKernelGGL((k_fit_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, level, it, mode, rp);
 }
 template <int PREC, int MT, int NT>
-static void eval_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int level, int it,
+static void eval_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int,
                              RunParams rp) {
-    hipLaunchKernelGGL((k_eval_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, level, it, rp);
+    hipLaunchKernelGGL((k_eval_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, rp);
 }
-template <int PREC, int MT, int NT>
-static void recon_mfma_launch(int n, size_t lds, hipStream_t s, const VolDesc* v, const Slab* sl, int level, int,
-                              RunParams rp) {
-    hipLaunchKernelGGL((k_recon_mfma<PREC, MT, NT>), dim3(n), dim3(kBlock), lds, s, v, sl, level, rp);
-}
-
 #define N4_ROW(F, P, M) {F<P, M, 1>, F<P, M, 2>, F<P, M, 3>, F<P, M, 4>}
 #define N4_TABLE(F) \
     {{N4_ROW(F, 1, 1), N4_ROW(F, 1, 2), N4_ROW(F, 1, 3), N4_ROW(F, 1, 4)}, \
      {N4_ROW(F, 2, 1), N4_ROW(F, 2, 2), N4_ROW(F, 2, 3), N4_ROW(F, 2, 4)}}
 static const FitLaunch kFitMfma[2][kMfmaMaxTiles][kMfmaMaxTiles] = N4_TABLE(fit_mfma_launch);
 static const EvalLaunch kEvalMfma[2][kMfmaMaxTiles][kMfmaMaxTiles] = N4_TABLE(eval_mfma_launch);
-static const EvalLaunch kReconMfma[2][kMfmaMaxTiles][kMfmaMaxTiles] = N4_TABLE(recon_mfma_launch);
 #undef N4_TABLE
 #undef N4_ROW
 
@@ -1426,18 +1535,16 @@ int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int, RunParam
     return last_launch_status();
 }
 
-int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunParams rp, void* stream) {
+int launch_level(const VolDesc* vols, int nvol, int level, RunParams rp, void* stream) {
     if (nvol <= 0) return 0;
-    hipLaunchKernelGGL(k_level, dim3(nvol), dim3(kBlock), lds_level(), (hipStream_t)stream, vols, level, prev_iters,
-                       rp);
+    hipLaunchKernelGGL(k_level, dim3(nvol), dim3(kBlock), lds_level(), (hipStream_t)stream, vols, level, rp);
     return last_launch_status();
 }
 
-int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
-                int32_t* dbg_idx, size_t lds, void* stream) {
+int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int step, RunParams rp, int32_t* dbg_idx,
+                size_t lds, void* stream) {
     if (nslab <= 0) return 0;
-    hipLaunchKernelGGL(k_hist, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, it, rp,
-                       dbg_idx);
+    hipLaunchKernelGGL(k_hist, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, step, rp, dbg_idx);
     return last_launch_status();
 }
 
@@ -1467,26 +1574,21 @@ int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mod
     return last_launch_status();
 }
 
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp, size_t lds,
-                int mt, int nt, void* stream) {
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, size_t lds, int mt, int nt,
+                void* stream) {
     if (nslab <= 0) return 0;
     if (rp.prec) {
         if (!mfma_ok(rp, mt, nt)) return (int)hipErrorInvalidValue;
-        kEvalMfma[rp.prec - 1][mt - 1][nt - 1](nslab, lds, (hipStream_t)stream, vols, slabs, level, it, rp);
+        kEvalMfma[rp.prec - 1][mt - 1][nt - 1](nslab, lds, (hipStream_t)stream, vols, slabs, 0, rp);
         return last_launch_status();
     }
-    hipLaunchKernelGGL(k_eval, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, it, rp);
+    hipLaunchKernelGGL(k_eval, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, rp);
     return last_launch_status();
 }
 
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, int mt,
-                 int nt, void* stream) {
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
+                 void* stream) {
     if (nslab <= 0) return 0;
-    if (rp.prec) {
-        if (!mfma_ok(rp, mt, nt)) return (int)hipErrorInvalidValue;
-        kReconMfma[rp.prec - 1][mt - 1][nt - 1](nslab, lds, (hipStream_t)stream, vols, slabs, level, 0, rp);
-        return last_launch_status();
-    }
     hipLaunchKernelGGL(k_recon, dim3(nslab), dim3(kBlock), lds, (hipStream_t)stream, vols, slabs, level, rp);
     return last_launch_status();
 }
@@ -1513,16 +1615,16 @@ int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int, RunParam
     return 0;
 }
 
-int launch_level(const VolDesc* vols, int nvol, int level, int prev_iters, RunParams rp, void*) {
+int launch_level(const VolDesc* vols, int nvol, int level, RunParams rp, void*) {
     unsigned char* b = emu_lds(lds_level());
-    for (int i = 0; i < nvol; ++i) body_level(vols, level, prev_iters, rp, i, b);
+    for (int i = 0; i < nvol; ++i) body_level(vols, level, rp, i, b);
     return 0;
 }
 
-int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp,
-                int32_t* dbg_idx, size_t lds, void*) {
+int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int step, RunParams rp, int32_t* dbg_idx,
+                size_t lds, void*) {
     unsigned char* b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_hist(vols, slabs, level, it, rp, dbg_idx, i, b);
+    for (int i = 0; i < nslab; ++i) body_hist(vols, slabs, step, rp, dbg_idx, i, b);
     return 0;
 }
 
@@ -1547,17 +1649,14 @@ int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mod
     return 0;
 }
 
-int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, RunParams rp, size_t lds,
-                int, int, void*) {
+int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, size_t lds, int, int, void*) {
     if (rp.prec) return 1;
     unsigned char* b = emu_lds(lds);
-    for (int i = 0; i < nslab; ++i) body_eval(vols, slabs, level, it, rp, i, b);
+    for (int i = 0; i < nslab; ++i) body_eval(vols, slabs, rp, i, b);
     return 0;
 }
 
-int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, int, int,
-                 void*) {
-    if (rp.prec) return 1;
+int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, void*) {
     unsigned char* b = emu_lds(lds);
     for (int i = 0; i < nslab; ++i) body_recon(vols, slabs, level, rp, i, b);
     return 0;

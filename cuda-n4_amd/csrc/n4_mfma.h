@@ -9,13 +9,15 @@
 //   fit   T1[y,cz]  = sum_z g[y,z] Pz[z,cz]          v_mfma_f32_16x16x32_bf16
 //         T2[cy,cz] = sum_y Py[y,cy] T1[y,cz]        v_mfma_f32_16x16x16_bf16
 //   eval  Bt[c,y]   = sum_cy A[cy,c] Wy[y,cy]        v_mfma_f32_16x16x16_bf16
-//         f[z,y]    = sum_c Wz[z,c] Bt[c,y]          v_mfma_f32_16x16x16_bf16
+//         d[z,y]    = sum_c Wz[z,c] Bt[c,y]          v_mfma_f32_16x16x16_bf16
 //
 // The accumulator layout of one product is the operand layout of the next
 // (lane l holds rows 4*(l/16)+i of column l%16), so the chains never leave
-// registers.  Inputs are rounded to BF16 (PREC 1) or split into hi + lo BF16
-// halves with the product hi*hi + hi*lo + lo*hi (PREC 2); accumulation is
-// FP32 throughout.  The voxel-side work (remap, masks, exp, FP64 CV sums)
+// registers.  The voxel-side operand (residual, x-contracted increment) is
+// rounded to BF16 (PREC 1) or split into hi + lo BF16 halves (PREC 2); the
+// B-spline basis factors are always hi + lo (their rounding would otherwise
+// be systematic); products dh*bh + dh*bl (+ dl*bh); accumulation is FP32
+// throughout.  The voxel-side work (remap, masks, exp, FP64 CV sums)
 // is identical to the FP32 kernels.
 //
 // A wave owns 16 rows (y) of the slab at a time; 4 waves per workgroup.
@@ -51,22 +53,22 @@ __device__ inline f32x4 mma_k16(i16x4 a, i16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// c += a*b with PREC 1 (hi only) or PREC 2 (hi*hi + lo*hi + hi*lo).
-template <int PREC>
-__device__ inline f32x4 mma_k32p(u16x8 ah, u16x8 al, u16x8 bh, u16x8 bl, f32x4 c) {
-    if constexpr (PREC == 2) {
-        c = mma_k32(al, bh, c);
-        c = mma_k32(ah, bl, c);
-    }
-    return mma_k32(ah, bh, c);
+// c += data * basis.  The B-spline basis factors (weights, w^3/S, w^2) are
+// the same for every voxel, so their rounding would be a systematic error:
+// they are always carried as hi + lo.  The voxel-side operand is one BF16
+// (PREC 1) or hi + lo (PREC 2).  Products: dh*bh + dh*bl (+ dl*bh).
+// DATA_A: the data operand is the instruction's A operand (else B).
+template <int PREC, bool DATA_A>
+__device__ inline f32x4 mma_k32p(u16x8 dh, u16x8 dl, u16x8 bh, u16x8 bl, f32x4 c) {
+    if constexpr (PREC == 2) c = DATA_A ? mma_k32(dl, bh, c) : mma_k32(bh, dl, c);
+    c = DATA_A ? mma_k32(dh, bl, c) : mma_k32(bl, dh, c);
+    return DATA_A ? mma_k32(dh, bh, c) : mma_k32(bh, dh, c);
 }
-template <int PREC>
-__device__ inline f32x4 mma_k16p(i16x4 ah, i16x4 al, i16x4 bh, i16x4 bl, f32x4 c) {
-    if constexpr (PREC == 2) {
-        c = mma_k16(al, bh, c);
-        c = mma_k16(ah, bl, c);
-    }
-    return mma_k16(ah, bh, c);
+template <int PREC, bool DATA_A>
+__device__ inline f32x4 mma_k16p(i16x4 dh, i16x4 dl, i16x4 bh, i16x4 bl, f32x4 c) {
+    if constexpr (PREC == 2) c = DATA_A ? mma_k16(dl, bh, c) : mma_k16(bh, dl, c);
+    c = DATA_A ? mma_k16(dh, bl, c) : mma_k16(bl, dh, c);
+    return DATA_A ? mma_k16(dh, bh, c) : mma_k16(bh, dh, c);
 }
 
 // Four accumulator floats as a K=16 operand (hi / lo).
@@ -143,10 +145,10 @@ HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, i
     const int zs = round_up(Z, 32) + 8, ys = round_up(Y, 16) + 4;
     L.Pz.stride = zs;
     L.Pz.hi = c.take<unsigned short>((size_t)NT * 16 * zs);
-    L.Pz.lo = prec == 2 ? c.take<unsigned short>((size_t)NT * 16 * zs) : nullptr;
+    L.Pz.lo = c.take<unsigned short>((size_t)NT * 16 * zs);
     L.Py.stride = ys;
     L.Py.hi = c.take<unsigned short>((size_t)MT * 16 * ys);
-    L.Py.lo = prec == 2 ? c.take<unsigned short>((size_t)MT * 16 * ys) : nullptr;
+    L.Py.lo = c.take<unsigned short>((size_t)MT * 16 * ys);
     L.red = c.take<float>((size_t)MT * NT * 256);
     if (o) *o = L;
     return align16(c.off);
@@ -158,7 +160,7 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     const VolState* st = V.st;
-    if (mode == kFitNum && st->done) return;
+    if (!fit_step_mode(st, &mode, &level, &it)) return;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
@@ -166,9 +168,12 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     FitMfmaLds S;
     carve_fit_mfma(smem, nb, Y, Z, MT, NT, PREC, &S);
     const bool num = (mode == kFitNum);
+    const bool den = (mode == kFitDen || mode == kFitDenScatter);
+    const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
+    const int gmode = den ? (int)kFitDen : mode;
     const bool degenerate = num && st->degenerate;
     const double bmin = st->bin_min, slope = st->slope;
-    const int kf = (mode == kFitDen) ? 2 : 1;  // q = w^2 for DEN, p = w^3/S otherwise
+    const int kf = den ? 2 : 1;  // q = w^2 for the denominator, p = w^3/S otherwise
     const int x = sl.x;
     const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
     const int Zp = round_up(Z, 32), Yp = round_up(Y, 16);
@@ -195,11 +200,11 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
             UNROLL for (int j = 0; j < 8; ++j) {
                 const bool ok = yok && (zb + j) < Z;
                 ms[j] = ok ? V.wmask[row + zb + j] : 0;
-                xs[j] = (ok && mode != kFitDen) ? V.logU[row + zb + j] : 0.0f;
+                xs[j] = (ok && !den) ? V.logU[row + zb + j] : 0.0f;
             }
             u16x8 ah, al;
             UNROLL for (int j = 0; j < 8; ++j) {
-                const float gv = fit_input(mode, ms[j], xs[j], degenerate, S.E, bmin, slope, nb);
+                const float gv = fit_input(gmode, ms[j], xs[j], degenerate, S.E, bmin, slope, nb);
                 unsigned short h, l = 0;
                 if constexpr (PREC == 2)
                     bf16_split(gv, h, l);
@@ -210,8 +215,8 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
             }
             UNROLL for (int n = 0; n < NT; ++n) {
                 const u16x8 bh = S.Pz.get8(S.Pz.hi, n * 16 + r16, zb);
-                const u16x8 bl = (PREC == 2) ? S.Pz.get8(S.Pz.lo, n * 16 + r16, zb) : bh;
-                t1[n] = mma_k32p<PREC>(ah, al, bh, bl, t1[n]);
+                const u16x8 bl = S.Pz.get8(S.Pz.lo, n * 16 + r16, zb);
+                t1[n] = mma_k32p<PREC, true>(ah, al, bh, bl, t1[n]);
             }
         }
         // t1[n]: lane holds T1[y0 + 4*g4 + i][16n + r16] = the K=16 operand (k = y)
@@ -220,8 +225,8 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
             to_op4<PREC>(t1[n], bh, bl);
             UNROLL for (int m = 0; m < MT; ++m) {
                 const i16x4 a_h = S.Py.get4(S.Py.hi, m * 16 + r16, y0 + 4 * g4);
-                const i16x4 a_l = (PREC == 2) ? S.Py.get4(S.Py.lo, m * 16 + r16, y0 + 4 * g4) : a_h;
-                acc[m][n] = mma_k16p<PREC>(a_h, a_l, bh, bl, acc[m][n]);
+                const i16x4 a_l = S.Py.get4(S.Py.lo, m * 16 + r16, y0 + 4 * g4);
+                acc[m][n] = mma_k16p<PREC, false>(bh, bl, a_h, a_l, acc[m][n]);
             }
         }
     }
@@ -242,34 +247,35 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
         const int m = tile / NT, n = tile - (tile / NT) * NT;
         const int cy = m * 16 + 4 * (ln >> 4) + i, cz = n * 16 + (ln & 15);
         if (cy < Cy && cz < Cz) {
-            if (num)
-                scatter_numerator(V, LD, ex, it, cy * Cz + cz, S.red[e]);
+            if (scatter)
+                scatter_numerator(V, LD, ex, num ? (it & 1) : 0, den, cy * Cz + cz, S.red[e]);
             else
                 out[cy * Cz + cz] = S.red[e];
         }
     }
 }
 
-// ------------------------------------------------------- eval and recon --
-// Shared by the working-grid evaluation of the lattice increment (logU
-// update and CV sums) and the full-resolution reconstruction of the final
-// field (outputs).  Four control planes along x feed each slab:
-// A[cy,cz] = sum_k wx[k] plane_k[cy,cz] (FP32), then the (y, z) chain runs on
-// MFMA.
+// ------------------------------------------------------------------ eval --
+// Working-grid evaluation of the lattice increment (logU update and CV
+// sums).  Four control planes along x feed each slab: A[cy,cz] =
+// sum_k wx[k] plane_k[cy,cz] (FP32), then the (y, z) chain runs on MFMA.
+// (The final full-resolution reconstruction streams 12 B per voxel from
+// HBM and stays on the FP32 kernel in every mode: an MFMA form with its
+// BF16 tables in LDS ran 5x slower there, limited by occupancy.)
 struct EvalMfmaLds {
-    float* LR;   // [4][Cy*Cz]  lattice increment planes (eval only)
+    float* LR;   // [4][Cy*Cz]  lattice increment planes
     float* A;    // [Cy*Cz]     x-contracted planes
     Tab At;      // [NT*16][MT*16 + 4]  A^T: row = cz, column = cy
     Tab Wy;      // [Yrows][MT*16 + 4]  row = y (slab-local), column = cy
     Tab Wz;      // [Zp16][NT*16 + 4]   row = z, column = cz
-    double* red; // [4*kBlock]  (eval only)
+    double* red; // [4*kBlock]
 };
 
 HD size_t carve_eval_mfma(unsigned char* base, int Yrows, int Z, int Cy, int Cz, int MT, int NT, int prec,
-                          bool eval, EvalMfmaLds* o) {
+                          EvalMfmaLds* o) {
     Carve c{base, 0};
     EvalMfmaLds L;
-    L.LR = eval ? c.take<float>((size_t)4 * Cy * Cz) : nullptr;
+    L.LR = c.take<float>((size_t)4 * Cy * Cz);
     L.A = c.take<float>((size_t)Cy * Cz);
     const int as = MT * 16 + 4, zs = NT * 16 + 4;
     L.At.stride = as;
@@ -278,12 +284,12 @@ HD size_t carve_eval_mfma(unsigned char* base, int Yrows, int Z, int Cy, int Cz,
     const int yr = round_up(Yrows, 16);
     L.Wy.stride = as;
     L.Wy.hi = c.take<unsigned short>((size_t)yr * as);
-    L.Wy.lo = prec == 2 ? c.take<unsigned short>((size_t)yr * as) : nullptr;
+    L.Wy.lo = c.take<unsigned short>((size_t)yr * as);
     const int zr = round_up(Z, 16);
     L.Wz.stride = zs;
     L.Wz.hi = c.take<unsigned short>((size_t)zr * zs);
-    L.Wz.lo = prec == 2 ? c.take<unsigned short>((size_t)zr * zs) : nullptr;
-    L.red = eval ? c.take<double>((size_t)4 * kBlock) : nullptr;
+    L.Wz.lo = c.take<unsigned short>((size_t)zr * zs);
+    L.red = c.take<double>((size_t)4 * kBlock);
     if (o) *o = L;
     return align16(c.off);
 }
@@ -323,8 +329,8 @@ __device__ inline void eval_mfma_rows(const EvalMfmaLds& S, int r0, int r16, int
             const i16x4 a_h = S.At.get4(S.At.hi, n * 16 + r16, k * 16 + 4 * g4);
             const i16x4 a_l = (PREC == 2) ? S.At.get4(S.At.lo, n * 16 + r16, k * 16 + 4 * g4) : a_h;
             const i16x4 w_h = S.Wy.get4(S.Wy.hi, r0 + r16, k * 16 + 4 * g4);
-            const i16x4 w_l = (PREC == 2) ? S.Wy.get4(S.Wy.lo, r0 + r16, k * 16 + 4 * g4) : w_h;
-            b = mma_k16p<PREC>(a_h, a_l, w_h, w_l, b);
+            const i16x4 w_l = S.Wy.get4(S.Wy.lo, r0 + r16, k * 16 + 4 * g4);
+            b = mma_k16p<PREC, true>(a_h, a_l, w_h, w_l, b);
         }
         to_op4<PREC>(b, bh[n], bl[n]);
     }
@@ -337,8 +343,8 @@ __device__ inline f32x4 eval_mfma_tile(const EvalMfmaLds& S, int z0, int r16, in
     f32x4 f = f32x4{0, 0, 0, 0};
     UNROLL for (int n = 0; n < NT; ++n) {
         const i16x4 w_h = S.Wz.get4(S.Wz.hi, z0 + r16, n * 16 + 4 * g4);
-        const i16x4 w_l = (PREC == 2) ? S.Wz.get4(S.Wz.lo, z0 + r16, n * 16 + 4 * g4) : w_h;
-        f = mma_k16p<PREC>(w_h, w_l, bh[n], bl[n], f);
+        const i16x4 w_l = S.Wz.get4(S.Wz.lo, z0 + r16, n * 16 + 4 * g4);
+        f = mma_k16p<PREC, false>(bh[n], bl[n], w_h, w_l, f);
     }
     return f;
 }
@@ -346,16 +352,21 @@ __device__ inline f32x4 eval_mfma_tile(const EvalMfmaLds& S, int z0, int r16, in
 // Field increment d on the working grid, logU -= d, CV sums and logU
 // min/max per slab (same contract as the FP32 evaluation kernel).
 template <int PREC, int MT, int NT>
-__device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, int level, int it, RunParams rp, int bid,
-                               unsigned char* smem) {
+__device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    if (V.st->done) return;
+    const int kind = V.st->s_kind;
+    if (kind == kStepTrans) {
+        finalize_denominator(V, V.st->s_lvl, sl.x, (int)threadIdx.x);
+        return;
+    }
+    if (kind != kStepIter) return;
+    const int level = V.st->s_lvl, it = V.st->s_it;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
     EvalMfmaLds S;
-    carve_eval_mfma(smem, Y, Z, Cy, Cz, MT, NT, PREC, true, &S);
+    carve_eval_mfma(smem, Y, Z, Cy, Cz, MT, NT, PREC, &S);
     const int x = sl.x;
     const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
     const AxisEntry ex = LD.ax[0][x];
@@ -402,46 +413,6 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, int level
         sp[1] = S.red[kBlock];
         sp[2] = S.red[2 * kBlock];
         sp[3] = S.red[3 * kBlock];
-    }
-}
-
-template <int PREC, int MT, int NT>
-__device__ void body_recon_mfma(const VolDesc* vols, const Slab* slabs, int level, RunParams rp, int bid,
-                                unsigned char* smem) {
-    const Slab sl = slabs[bid];
-    const VolDesc& V = vols[sl.vol];
-    const LevelDesc& LD = V.lev[level];
-    const int Y = V.F[1], Z = V.F[2];
-    const int Cy = LD.C[1], Cz = LD.C[2];
-    EvalMfmaLds S;
-    carve_eval_mfma(smem, sl.ny, Z, Cy, Cz, MT, NT, PREC, false, &S);
-    const int x = sl.x;
-    const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
-    const AxisEntry ex = V.fax[0][x];
-    eval_mfma_setup<MT, NT>(S, ex, V.latf + (size_t)ex.b * Cy * Cz, V.fax[1], sl.y0, sl.ny, V.fax[2], Z, Cy, Cz);
-    for (int r0 = 16 * wv; r0 < sl.ny; r0 += 64) {
-        i16x4 bh[NT], bl[NT];
-        eval_mfma_rows<PREC, MT, NT>(S, r0, r16, g4, bh, bl);
-        const int r = r0 + r16;
-        const bool yok = r < sl.ny;
-        const size_t row = ((size_t)x * Y + sl.y0 + (yok ? r : 0)) * Z;
-        for (int z0 = 0; z0 < Z; z0 += 16) {
-            const f32x4 f = eval_mfma_tile<PREC, NT>(S, z0, r16, g4, bh, bl);
-            const int zb = z0 + 4 * g4;
-            float xin[4];
-            UNROLL for (int i = 0; i < 4; ++i)
-                xin[i] = (V.out_norm && yok && (zb + i) < Z) ? V.im[row + zb + i] : 0.0f;
-            UNROLL for (int i = 0; i < 4; ++i) {
-                if (!yok || zb + i >= Z) continue;
-                const size_t v = row + zb + i;
-                if (V.out_logbias) V.out_logbias[v] = f[i];
-                if (V.out_norm || V.out_bias) {
-                    const float ef = expf(f[i]);
-                    if (V.out_bias) V.out_bias[v] = ef;
-                    if (V.out_norm) V.out_norm[v] = ((xin[i] <= rp.low_f) ? 0.0f : xin[i]) / ef;
-                }
-            }
-        }
     }
 }
 

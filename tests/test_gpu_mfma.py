@@ -1,11 +1,13 @@
 """BF16 and split-BF16 MFMA forms of the separable B-spline contractions
-(fit numerator/denominator, working-grid evaluation, full-resolution
-reconstruction) against the FP64 oracle.
+(fit numerator and denominator, working-grid evaluation of the field
+increments) against the FP64 oracle.  The final full-resolution
+reconstruction is the FP32 kernel in every mode.
 
 Tolerances on the log bias field (max abs over masked voxels):
-  * split-BF16 (hi + lo halves, ~16 significant bits per operand): 1e-4,
-    the same bar as the FP32 path;
-  * BF16 (8 significant bits per operand): 1e-3, the batched-path bar.
+  * split-BF16 (voxel operands and basis factors as hi + lo BF16 halves,
+    ~16 significant bits): 1e-4, the same bar as the FP32 path;
+  * BF16 (voxel operands in BF16, basis factors hi + lo): 1e-3, the
+    batched-path bar.
 Where the convergence test decides the iteration count, a BF16 run may stop
 one iteration earlier or later than the FP64 oracle; those cases compare
 the fields without requiring equal counts."""
@@ -49,9 +51,8 @@ def test_split_golden_trajectory(gpu, golden):
 
 @pytest.mark.parametrize("shrink", [2, 3])
 def test_split_shrink_partial_tiles(gpu, shrink):
-    """Edge tiles everywhere: 75x62x54 leaves partial 16-row, 16- and 32-wide
-    z tiles on both the working and the full-resolution grid, and several
-    reconstruction row tiles per slab."""
+    """Edge tiles everywhere: 75x62x54 leaves partial 16-row tiles and
+    partial 16- and 32-wide z tiles on the working grid."""
     im, m, _ = PC.small_phantom((75, 62, 54), seed=9)
     print(f"split_bf16: shrink {shrink} on 75x62x54", flush=True)
     PC.check_n4_vs_oracle(gpu, im, m, 3, [12] * 3, 0.0, shrink=shrink, fixed=True, precision="split_bf16")
