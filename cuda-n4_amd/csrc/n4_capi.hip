@@ -243,6 +243,16 @@ static uint64_t magic_div(uint32_t d) { return ((uint64_t)1 << kDivShift) / d + 
 // Working voxels per prep tile and full-resolution voxels per recon tile:
 // enough workgroups to fill the device even for one volume.
 static const int kPrepTileVox = 1024;
+// Working voxels per tile of the step kernels (env N4_STEP_TILE_VOX
+// overrides; read per call).
+static const int kStepTileVox = 8192;
+static int step_tile_vox() {
+    if (const char* e = getenv("N4_STEP_TILE_VOX")) {
+        const int v = atoi(e);
+        if (v >= 16) return v;
+    }
+    return kStepTileVox;
+}
 static const int kReconTileVox = 8192;
 
 static int rows_for(int rows, int cols, int target) {
@@ -253,8 +263,10 @@ static int rows_for(int rows, int cols, int target) {
 
 // Builds the plan and fills h->staging with the upload region.  Pointers in
 // the descriptors are arena-relative until `rebase` adds the arena base.
+// tiled: cut working slabs into row tiles for the step kernels (the
+// correction); stage calls keep whole slabs (their partials are per slab).
 static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, Plan* P,
-                            bool need_tables_full = true) {
+                            bool need_tables_full = true, bool tiled = false) {
     const int nv = (int)in.size();
     const int L = (int)p->number_of_fitting_levels;
     P->nvol = nv;
@@ -268,10 +280,11 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         int W[3];
         int C[kMaxLevels][3];
         size_t Lmax, part_stride;
-        int prep_rows, prep_tiles_x, recon_rows, recon_tiles_x;
+        int prep_rows, prep_tiles_x, recon_rows, recon_tiles_x, wtile_rows, wtiles_x;
     };
     std::vector<VInfo> vi(nv);
     int nws = 0, nfs = 0, nps = 0;
+    const int tile_vox = step_tile_vox();
     for (int v = 0; v < nv; ++v) {
         VInfo& I = vi[v];
         for (int d = 0; d < 3; ++d) I.W[d] = (in[v].F[d] + in[v].s - 1) / in[v].s;
@@ -284,7 +297,14 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         I.prep_tiles_x = (I.W[1] + I.prep_rows - 1) / I.prep_rows;
         I.recon_rows = rows_for(in[v].F[1], in[v].F[2], kReconTileVox);
         I.recon_tiles_x = (in[v].F[1] + I.recon_rows - 1) / I.recon_rows;
-        nws += I.W[0];
+        I.wtile_rows = I.W[1];
+        if (tiled && (size_t)I.W[1] * I.W[2] > (size_t)tile_vox) {
+            // multiples of 16 rows (one MFMA row tile per wave)
+            const int r = (tile_vox / std::max(I.W[2], 1) + 15) / 16 * 16;
+            I.wtile_rows = std::min(I.W[1], std::max(r, 16));
+        }
+        I.wtiles_x = (I.W[1] + I.wtile_rows - 1) / I.wtile_rows;
+        nws += I.W[0] * I.wtiles_x;
         nps += I.W[0] * I.prep_tiles_x;
         nfs += in[v].F[0] * I.recon_tiles_x;
     }
@@ -335,7 +355,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
         o.numfx = lay.take(sizeof(uint64_t) * 2 * I.Lmax);
         o.hist = lay.take(sizeof(uint64_t) * nb);
-        o.spart = lay.take(sizeof(double) * 4 * I.W[0]);
+        o.spart = lay.take(sizeof(double) * 4 * I.W[0] * I.wtiles_x);
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
         o.etab = lay.take(sizeof(double) * nb);
     }
@@ -395,6 +415,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.ppart = (double*)(base + o.ppart);
         D.prep_rows = I.prep_rows;
         D.prep_tiles_x = I.prep_tiles_x;
+        D.wtile_rows = I.wtile_rows;
+        D.wtiles_x = I.wtiles_x;
         D.mW2 = magic_div((uint32_t)I.W[2]);
         D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
@@ -427,7 +449,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         }
         P->desc[v] = D;
         for (int x = 0; x < I.W[0]; ++x) {
-            ws[iws++] = Slab{v, x, 0, I.W[1]};
+            for (int y0 = 0; y0 < I.W[1]; y0 += I.wtile_rows)
+                ws[iws++] = Slab{v, x, y0, std::min(I.wtile_rows, I.W[1] - y0)};
             for (int y0 = 0; y0 < I.W[1]; y0 += I.prep_rows)
                 ps[ips++] = Slab{v, x, y0, std::min(I.prep_rows, I.W[1] - y0)};
         }
@@ -733,7 +756,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     N4_CHECK_HIP(hipEventRecord(ev.a, (hipStream_t)stream));
 #endif
     Plan P;
-    s = build_plan(h, in, p, &P);
+    s = build_plan(h, in, p, &P, true, true);
     if (s != N4_OK) return s;
     s = check_lds(P, true, true);
     if (s != N4_OK) return s;
@@ -754,7 +777,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             R.fs0 = fs;
             R.ps0 = ps;
             while (v < nv && gid[order[v]] == g) {
-                ws += P.desc[v].W[0];
+                ws += P.desc[v].W[0] * P.desc[v].wtiles_x;
                 fs += P.recon_tiles[v];
                 ps += P.desc[v].W[0] * P.desc[v].prep_tiles_x;
                 ++v;

@@ -103,6 +103,8 @@ struct VolDesc {
     int32_t part_stride;       // floats per slab in part (max Cy*Cz)
     int32_t prep_rows;         // working rows per prep tile
     int32_t prep_tiles_x;      // prep tiles per working slab
+    int32_t wtile_rows;        // working rows per tile of the step kernels
+    int32_t wtiles_x;          // such tiles per working slab (spart has W0*wtiles_x entries)
     double* ppart;             // prep partials [W0 * prep_tiles_x][4]
     uint64_t mW2;              // magic divisor for the working z extent
     uint64_t mF2;              // magic divisor for the full-resolution z extent
@@ -128,8 +130,7 @@ struct RunParams {
     int32_t prec;     // n4_precision: 0 FP32, 1 BF16 MFMA, 2 split-BF16 MFMA
 };
 
-// One workgroup's tile: rows [y0, y0+ny) of slab x of volume vol.  The
-// working-grid iteration kernels always take whole slabs (y0 = 0).
+// One workgroup's tile: rows [y0, y0+ny) of slab x of volume vol.
 struct Slab {
     int32_t vol;
     int32_t x;

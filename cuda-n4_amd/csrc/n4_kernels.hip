@@ -84,6 +84,11 @@ HD void sincospi_d(double x, double* s, double* c) {
 }
 
 HD size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+
+// Index of a step-kernel tile within its volume (its slot in spart).
+HD int tile_id(const VolDesc& V, const Slab& sl) {
+    return sl.x * V.wtiles_x + (V.wtiles_x > 1 ? sl.y0 / V.wtile_rows : 0);
+}
 HD int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // floor(v / d) with the host-computed magic m = floor(2^40/d) + 1
@@ -400,7 +405,7 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
     int Lmax = 0;
     for (int l = 0; l < V.nlevels; ++l) Lmax = V.lev[l].L > Lmax ? V.lev[l].L : Lmax;
     BLOCK_FOR(t)
-    for (int xx = t; xx < V.W[0]; xx += kBlock) {
+    for (int xx = t; xx < V.W[0] * V.wtiles_x; xx += kBlock) {
         double* sp = V.spart + (size_t)xx * 4;
         sp[0] = 0.0;
         sp[1] = 0.0;
@@ -467,14 +472,14 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     const int mode = step < 0 ? kModeIter : st->mode;
     const int lvl = step < 0 ? 0 : st->lvl, it = step < 0 ? 0 : st->it;
     if (mode == kModeFinished) {
-        if (sl.x == 0) {
+        if (sl.x == 0 && sl.y0 == 0) {
             BLOCK_FOR(t)
             if (t == 0) st->s_kind = kStepSkip;
             BLOCK_END
         }
         return;
     }
-    reduce_partials(V.spart, V.W[0], red);
+    reduce_partials(V.spart, V.W[0] * V.wtiles_x, red);
     const double n = (double)st->n_mask_work;
     const double mn = red[2 * kBlock], mx = red[3 * kBlock];
     double cvv = 0.0;
@@ -495,7 +500,7 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     }
     const double slope = (mx - mn) / (double)(nb - 1);
     const bool degenerate = !(n > 0.0) || !(slope > 0.0);
-    if (sl.x == 0) {
+    if (sl.x == 0 && sl.y0 == 0) {
         BLOCK_FOR(t)
         if (t == 0) {
             if (level_end) {
@@ -517,8 +522,8 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
         BLOCK_END
     }
     if (kind != kStepIter || degenerate) return;
-    const int YZ = V.W[1] * V.W[2];
-    const size_t base = (size_t)sl.x * YZ;
+    const int YZ = sl.ny * V.W[2];
+    const size_t base = ((size_t)sl.x * V.W[1] + sl.y0) * V.W[2];
     BLOCK_FOR(t)
     for (int i = t; i < nb; i += kBlock) h[i] = 0ull;
     BLOCK_END
@@ -797,14 +802,15 @@ HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry
 // plane in the other numerator buffer for the next iteration, so each plane
 // is updated exactly once; planes without support have num = 0.  Runs
 // inside a thread phase (thread t of kBlock).
-HD void lattice_planes(const VolDesc& V, const LevelDesc& LD, double* lat, int x, const AxisEntry& ex, int it,
-                       double eps, float* LR, int t) {
+HD void lattice_planes(const VolDesc& V, const LevelDesc& LD, double* lat, const Slab& sl, const AxisEntry& ex,
+                       int it, double eps, float* LR, int t) {
+    const int x = sl.x;
     const int CyCz = LD.C[1] * LD.C[2];
     const uint64_t* acc = V.numfx[it & 1];
     uint64_t* next = V.numfx[(it + 1) & 1];
     for (int k = 0; k < 4; ++k) {
         const int cx = ex.b + k;
-        const bool owner = (x == LD.sup[0][2 * cx]);
+        const bool owner = (x == LD.sup[0][2 * cx] && sl.y0 == 0);
         for (int pr = t; pr < CyCz; pr += kBlock) {
             const size_t l = (size_t)cx * CyCz + pr;
             const double num = (double)(long long)acc[l] / kNumScale;
@@ -821,8 +827,10 @@ HD void lattice_planes(const VolDesc& V, const LevelDesc& LD, double* lat, int x
 // First step of a level: the owner block of each of slab x's control planes
 // moves the fixed-point denominator into den (FP64) and clears buffer 0 for
 // the level's first numerator.  Runs inside a thread phase.
-HD void finalize_denominator(const VolDesc& V, int level, int x, int t) {
+HD void finalize_denominator(const VolDesc& V, int level, const Slab& sl, int t) {
     const LevelDesc& LD = V.lev[level];
+    const int x = sl.x;
+    if (sl.y0 != 0) return;
     const AxisEntry& ex = LD.ax[0][x];
     const int CyCz = LD.C[1] * LD.C[2];
     for (int k = 0; k < 4; ++k) {
@@ -945,8 +953,9 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     for (int i = t; i < Cy * Cz; i += kBlock) S.T2[i] = 0.0f;
     BLOCK_END
     BLOCK_SYNC();
-    for (int y0 = 0; y0 < Y; y0 += R) {
-        const int ny = (Y - y0 < R) ? Y - y0 : R;
+    const int yend = sl.y0 + sl.ny;
+    for (int y0 = sl.y0; y0 < yend; y0 += R) {
+        const int ny = (yend - y0 < R) ? yend - y0 : R;
         const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int r = t; r < ny; r += kBlock) {
@@ -1106,7 +1115,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     const int kind = V.st->s_kind;
     if (kind == kStepTrans) {
         BLOCK_FOR(t)
-        finalize_denominator(V, V.st->s_lvl, sl.x, t);
+        finalize_denominator(V, V.st->s_lvl, sl, t);
         BLOCK_END
         return;
     }
@@ -1122,7 +1131,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     const int x = sl.x;
     const AxisEntry ex = LD.ax[0][x];
     BLOCK_FOR(t)
-    lattice_planes(V, LD, V.lat[level & 1], x, ex, it, rp.eps, S.LR, t);
+    lattice_planes(V, LD, V.lat[level & 1], sl, ex, it, rp.eps, S.LR, t);
     BLOCK_END
     BLOCK_SYNC();
     BLOCK_FOR(t)
@@ -1142,8 +1151,9 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     S.red[3 * kBlock + t] = -INFINITY;
     BLOCK_END
     BLOCK_SYNC();
-    for (int y0 = 0; y0 < Y; y0 += R) {
-        const int ny = (Y - y0 < R) ? Y - y0 : R;
+    const int yend = sl.y0 + sl.ny;
+    for (int y0 = sl.y0; y0 < yend; y0 += R) {
+        const int ny = (yend - y0 < R) ? yend - y0 : R;
         const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
@@ -1193,7 +1203,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     tree_reduce4(S.red, 0, 0, 1, 2);
     BLOCK_FOR(t)
     if (t == 0) {
-        double* sp = V.spart + (size_t)x * 4;
+        double* sp = V.spart + (size_t)tile_id(V, sl) * 4;
         sp[0] = S.red[0];
         sp[1] = S.red[kBlock];
         sp[2] = S.red[2 * kBlock];
@@ -1408,7 +1418,8 @@ size_t lds_fit(int nbins, const VolDesc& v, int level, int prec) {
     const LevelDesc& L = v.lev[level];
 #ifndef N4_EMULATE
     if (prec)
-        return carve_fit_mfma(nullptr, nbins, v.W[1], v.W[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec, nullptr);
+        return carve_fit_mfma(nullptr, nbins, v.wtile_rows, v.W[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec,
+                              nullptr);
 #endif
     (void)prec;
     return carve_fit(nullptr, nbins, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
@@ -1418,8 +1429,8 @@ size_t lds_eval(const VolDesc& v, int level, int prec) {
     const LevelDesc& L = v.lev[level];
 #ifndef N4_EMULATE
     if (prec)
-        return carve_eval_mfma(nullptr, v.W[1], v.W[2], L.C[1], L.C[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]), prec,
-                               nullptr);
+        return carve_eval_mfma(nullptr, v.wtile_rows, v.W[2], L.C[1], L.C[2], cp_tiles(L.C[1]), cp_tiles(L.C[2]),
+                               prec, nullptr);
 #endif
     (void)prec;
     return carve_eval(nullptr, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);

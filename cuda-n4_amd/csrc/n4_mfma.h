@@ -166,7 +166,7 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     const int Cy = LD.C[1], Cz = LD.C[2];
     const int nb = rp.nbins;
     FitMfmaLds S;
-    carve_fit_mfma(smem, nb, Y, Z, MT, NT, PREC, &S);
+    carve_fit_mfma(smem, nb, V.wtile_rows, Z, MT, NT, PREC, &S);
     const bool num = (mode == kFitNum);
     const bool den = (mode == kFitDen || mode == kFitDenScatter);
     const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
@@ -176,21 +176,22 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     const int kf = den ? 2 : 1;  // q = w^2 for the denominator, p = w^3/S otherwise
     const int x = sl.x;
     const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
-    const int Zp = round_up(Z, 32), Yp = round_up(Y, 16);
+    const int Yt = sl.ny;  // the tile's rows sl.y0 .. sl.y0 + Yt (Py rows are tile-local)
+    const int Zp = round_up(Z, 32), Yp = round_up(Yt, 16);
 
     if (num)
         for (int i = t; i < nb; i += kBlock) S.E[i] = V.etab[i];
     for (int z = t; z < Zp; z += kBlock) fill_axis(S.Pz, true, z, NT * 16, Cz, z < Z ? LD.ax[2] + z : nullptr, kf);
-    for (int y = t; y < Yp; y += kBlock) fill_axis(S.Py, true, y, MT * 16, Cy, y < Y ? LD.ax[1] + y : nullptr, kf);
+    for (int y = t; y < Yp; y += kBlock)
+        fill_axis(S.Py, true, y, MT * 16, Cy, y < Yt ? LD.ax[1] + sl.y0 + y : nullptr, kf);
     __syncthreads();
 
     f32x4 acc[MT][NT];
     UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0, 0, 0, 0};
 
-    for (int y0 = 16 * wv; y0 < Y; y0 += 64) {
-        const int y = y0 + r16;
-        const bool yok = y < Y;
-        const size_t row = ((size_t)x * Y + (yok ? y : 0)) * Z;
+    for (int y0 = 16 * wv; y0 < Yt; y0 += 64) {
+        const bool yok = y0 + r16 < Yt;
+        const size_t row = ((size_t)x * Y + sl.y0 + (yok ? y0 + r16 : 0)) * Z;
         f32x4 t1[NT];
         UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
         for (int z0 = 0; z0 < Z; z0 += 32) {
@@ -357,7 +358,7 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
     const VolDesc& V = vols[sl.vol];
     const int kind = V.st->s_kind;
     if (kind == kStepTrans) {
-        finalize_denominator(V, V.st->s_lvl, sl.x, (int)threadIdx.x);
+        finalize_denominator(V, V.st->s_lvl, sl, (int)threadIdx.x);
         return;
     }
     if (kind != kStepIter) return;
@@ -366,20 +367,20 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
     EvalMfmaLds S;
-    carve_eval_mfma(smem, Y, Z, Cy, Cz, MT, NT, PREC, &S);
+    carve_eval_mfma(smem, V.wtile_rows, Z, Cy, Cz, MT, NT, PREC, &S);
     const int x = sl.x;
     const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
     const AxisEntry ex = LD.ax[0][x];
-    lattice_planes(V, LD, V.lat[level & 1], x, ex, it, rp.eps, S.LR, t);
+    lattice_planes(V, LD, V.lat[level & 1], sl, ex, it, rp.eps, S.LR, t);
     __syncthreads();
-    eval_mfma_setup<MT, NT>(S, ex, S.LR, LD.ax[1], 0, Y, LD.ax[2], Z, Cy, Cz);
+    const int Yt = sl.ny;  // the tile's rows (Wy rows are tile-local)
+    eval_mfma_setup<MT, NT>(S, ex, S.LR, LD.ax[1], sl.y0, Yt, LD.ax[2], Z, Cy, Cz);
     double s1 = 0.0, s2 = 0.0, mn = INFINITY, mx = -INFINITY;
-    for (int y0 = 16 * wv; y0 < Y; y0 += 64) {
+    for (int y0 = 16 * wv; y0 < Yt; y0 += 64) {
         i16x4 bh[NT], bl[NT];
         eval_mfma_rows<PREC, MT, NT>(S, y0, r16, g4, bh, bl);
-        const int y = y0 + r16;
-        const bool yok = y < Y;
-        const size_t row = ((size_t)x * Y + (yok ? y : 0)) * Z;
+        const bool yok = y0 + r16 < Yt;
+        const size_t row = ((size_t)x * Y + sl.y0 + (yok ? y0 + r16 : 0)) * Z;
         for (int z0 = 0; z0 < Z; z0 += 16) {
             const f32x4 d = eval_mfma_tile<PREC, NT>(S, z0, r16, g4, bh, bl);
             const int zb = z0 + 4 * g4;
@@ -408,7 +409,7 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
     S.red[3 * kBlock + t] = mx;
     tree_reduce4(S.red, 0, 0, 1, 2);
     if (t == 0) {
-        double* sp = V.spart + (size_t)x * 4;
+        double* sp = V.spart + (size_t)tile_id(V, sl) * 4;
         sp[0] = S.red[0];
         sp[1] = S.red[kBlock];
         sp[2] = S.red[2 * kBlock];

@@ -60,3 +60,12 @@ def test_mfma_precisions_need_the_device_build(emu):
     for prec in ("bf16", "split_bf16"):
         with pytest.raises(n4amd.N4Error, match="FP32 kernels only"):
             emu.correct(im, m, levels=2, max_iter=[2, 2], precision=prec)
+
+
+def test_n4_row_tiles(emu, monkeypatch):
+    """Step kernels over row tiles of the working slabs (16-row tiles on a
+    46x38x34 grid: 3 tiles per slab, the last one partial): the same parity
+    with the FP64 oracle as whole slabs."""
+    monkeypatch.setenv("N4_STEP_TILE_VOX", "256")
+    im, m, _ = PC.small_phantom((46, 38, 34), seed=9)
+    PC.check_n4_vs_oracle(emu, im, m, 3, [8] * 3, 0.0, fixed=True)

@@ -154,3 +154,13 @@ def test_invalid_arguments_raise(gpu):
         gpu.correct(im, m, ncpt=3)
     with pytest.raises(n4amd.N4Error):
         gpu.correct(im, m, nbins=1)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split_bf16"])
+def test_n4_row_tiles(gpu, monkeypatch, precision):
+    """Step kernels over 16-row tiles of the working slabs (75x62x54: four
+    tiles per slab, the last one partial) keep the oracle parity."""
+    monkeypatch.setenv("N4_STEP_TILE_VOX", "512")
+    im, m, _ = PC.small_phantom((75, 62, 54), seed=9)
+    print(f"{precision}: 16-row tiles on 75x62x54", flush=True)
+    PC.check_n4_vs_oracle(gpu, im, m, 3, [10] * 3, 0.0, fixed=True, precision=precision)
