@@ -125,6 +125,9 @@ def main():
                     help="precision of the headline batched figure")
     ap.add_argument("--precisions", default="fp32,bf16,split_bf16",
                     help="batched precisions timed (comma list; must include --precision)")
+    ap.add_argument("--schedule", default="autotune", choices=["autotune", "default", "direct"],
+                    help="launch schedule of the batched runs: the library's autotuner (per precision, before "
+                         "the warmup), its defaults, or its defaults without step graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -206,6 +209,18 @@ def main():
     runs = {}
     for prec in precs:
         p = n4amd.default_params(precision=prec, **base_kw)
+        lib.set_tuning(reset=True)
+        t_tune = 0.0
+        if args.schedule == "autotune":
+            best = n4amd.Tuning()
+            t0 = time.perf_counter()
+            lib._check(lib.L.n4_autotune_batch(lib.h, arr, nvol, C.byref(p), C.byref(best), None))
+            t_tune = time.perf_counter() - t0
+            log(f"[{prec}] autotuned schedule {best.as_dict()} in {t_tune:.1f}s")
+        elif args.schedule == "direct":
+            lib.set_tuning(graph_steps="off")
+        schedule = lib.get_tuning().as_dict()
+        schedule.pop("ms", None)
         # warmup with every kernel timed: picks the dominant kernel
         lib.profile_enable(n4amd.KERNELS)
         for _ in range(max(1, args.warmup)):
@@ -241,7 +256,7 @@ def main():
                                            for l in range(cfg["levels"])],
              "dominant": dominant, "dom_ms": dom_ms, "dom_n": dom_n, "dom_bytes": dom_bytes, "achieved": achieved,
              "breakdown": {k: round(v[0] / max(1, args.warmup), 4) for k, v in breakdown.items()},
-             "err": sample_errors(its)}
+             "err": sample_errors(its), "schedule": schedule, "autotune_s": round(t_tune, 2)}
         runs[prec] = R
         log(f"[{prec}] batched: {R['ms_per_step']:.3f} ms/step, {R['volumes_per_s']:.1f} volumes/s; dominant "
             f"{dominant}: {dom_ms / max(1, dom_n) * 1e3:.2f} us avg over {dom_n} launches, {achieved:.1f} GB/s; "
@@ -272,7 +287,13 @@ def main():
         "iterations_per_level_mean": H["iterations_per_level_mean"],
         "precision": prec_text[args.precision] + "; FP64 histogram range, Wiener step, lattice update and CV sums",
         "roofline_note": "timed steps run without events; the dominant kernel's average duration comes from the "
-                         "same number of steps run again with events around each of its launches",
+                         "same number of steps run again with events around each of its launches (direct "
+                         "launches: step graphs are not used while kernels are timed with events)",
+        "schedule": H["schedule"],
+        "schedule_source": {"autotune": "n4_autotune_batch on this workload (groups x tile voxels x graph steps), "
+                                        "run before the warmup",
+                            "default": "library defaults", "direct": "library defaults, no step graphs"}[
+                                            args.schedule],
     }
     dominant, dom_n = H["dominant"], H["dom_n"]
     traffic, traffic_src = None, None
@@ -294,11 +315,13 @@ def main():
     result["kernel_breakdown_ms_per_step"] = H["breakdown"]
     result["batched"] = {q: {"volumes_per_s": round(R["volumes_per_s"], 3), "ms_per_step": round(R["ms_per_step"], 4),
                              "mvoxel_iter_per_s": round(R["mvoxel_iter_per_s"], 2),
-                             "logbias_err_vs_fp64": R["err"], "kernel_ms_per_step": R["breakdown"]}
+                             "logbias_err_vs_fp64": R["err"], "kernel_ms_per_step": R["breakdown"],
+                             "schedule": R["schedule"], "autotune_s": R["autotune_s"]}
                          for q, R in runs.items()}
     if H["err"] is not None:
         result["logbias_err_vs_fp64"] = H["err"]
     p_head = n4amd.default_params(precision=args.precision, **base_kw)
+    lib.set_tuning(**{k: v for k, v in H["schedule"].items()})
 
     # host-buffer rate: pageable host image + mask up, normalised + log bias down
     lib.profile_enable([])
@@ -324,6 +347,7 @@ def main():
     if not args.no_plain:
         p32 = n4amd.default_params(precision="fp32", **base_kw)
         rep1 = n4amd.Report()
+        lib.set_tuning(reset=True)  # the drop-in as shipped: default schedule
 
         def plain_pass():
             for i in range(nvol):

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -100,6 +101,16 @@ struct n4_handle {
     uint32_t prof_mask = 0;
     double prof_ms[N4_K_COUNT] = {0};
     uint64_t prof_n[N4_K_COUNT] = {0};
+    n4_tuning tune{};  // launch schedule; 0 fields = defaults
+#ifndef N4_EMULATE
+    // Captured chunks of iteration steps, one per group stream, reused while
+    // their launch arguments (the key) stay the same.
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<unsigned char> key;
+    } sgraph[kMaxGroups];
+    hipStream_t cap_stream = nullptr;
+#endif
 };
 
 // Early exit: each histogram pass writes the volume's progress word (level,
@@ -243,17 +254,58 @@ static uint64_t magic_div(uint32_t d) { return ((uint64_t)1 << kDivShift) / d + 
 // Working voxels per prep tile and full-resolution voxels per recon tile:
 // enough workgroups to fill the device even for one volume.
 static const int kPrepTileVox = 1024;
-// Working voxels per tile of the step kernels (env N4_STEP_TILE_VOX
-// overrides; read per call).
+// Default working voxels per tile of the step kernels.
 static const int kStepTileVox = 8192;
-static int step_tile_vox() {
-    if (const char* e = getenv("N4_STEP_TILE_VOX")) {
-        const int v = atoi(e);
-        if (v >= 16) return v;
-    }
-    return kStepTileVox;
-}
 static const int kReconTileVox = 8192;
+// Default iteration steps per captured graph, and the largest chunk.
+static const int kGraphSteps = 8;
+static const int kMaxGraphSteps = 64;
+
+// Launch schedule of one correction call (n4_tuning resolved).
+struct Schedule {
+    int groups;       // volume groups (streams)
+    int tile_vox;     // working voxels per step-kernel tile
+    int graph_steps;  // steps per captured graph; 0 = direct launches
+};
+
+static int env_int(const char* name, int dflt) {
+    if (const char* e = getenv(name)) {
+        char* end = nullptr;
+        const long v = strtol(e, &end, 10);
+        if (end != e) return (int)v;
+    }
+    return dflt;
+}
+
+// Handle setting, else environment, else default.  Groups: each group runs
+// its own chain of steps on its own stream (group 0 on the caller's
+// stream), so the small per-volume kernels of one group overlap the slab
+// kernels of the others and a group whose volumes converge stops without
+// waiting for the rest.
+static Schedule resolve_schedule(const n4_handle* h, int nvol) {
+    Schedule s;
+    int g = h->tune.groups ? (int)h->tune.groups : env_int("N4_GROUPS", nvol >= 8 ? 4 : (nvol >= 4 ? 2 : 1));
+    g = std::min(std::min(g, kMaxGroups), nvol);
+    s.groups = std::max(g, 1);
+    const int t = h->tune.step_tile_voxels ? (int)std::min<uint32_t>(h->tune.step_tile_voxels, 1u << 30)
+                                           : env_int("N4_STEP_TILE_VOX", kStepTileVox);
+    s.tile_vox = std::max(t, 16);
+    int k;
+    if (h->tune.graph_steps == N4_GRAPH_OFF)
+        k = 0;
+    else if (h->tune.graph_steps)
+        k = (int)std::min<uint32_t>(h->tune.graph_steps, kMaxGraphSteps);
+    else
+        k = env_int("N4_GRAPH_STEPS", kGraphSteps);
+    s.graph_steps = std::min(std::max(k, 0), kMaxGraphSteps);
+    return s;
+}
+
+// Copies of each volume's global histogram: the tiles of a correction
+// spread their atomic flushes over up to kHistCopies copies (stage calls
+// keep one, which they read back directly).
+static const int kHistCopies = 16;
+static int hist_copies(int tiles, bool tiled) { return tiled ? std::max(1, std::min(kHistCopies, tiles)) : 1; }
 
 static int rows_for(int rows, int cols, int target) {
     int r = target / (cols > 0 ? cols : 1);
@@ -266,7 +318,7 @@ static int rows_for(int rows, int cols, int target) {
 // tiled: cut working slabs into row tiles for the step kernels (the
 // correction); stage calls keep whole slabs (their partials are per slab).
 static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, Plan* P,
-                            bool need_tables_full = true, bool tiled = false) {
+                            bool need_tables_full = true, bool tiled = false, int tile_vox = kStepTileVox) {
     const int nv = (int)in.size();
     const int L = (int)p->number_of_fitting_levels;
     P->nvol = nv;
@@ -284,7 +336,6 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     };
     std::vector<VInfo> vi(nv);
     int nws = 0, nfs = 0, nps = 0;
-    const int tile_vox = step_tile_vox();
     for (int v = 0; v < nv; ++v) {
         VInfo& I = vi[v];
         for (int d = 0; d < 3; ++d) I.W[d] = (in[v].F[d] + in[v].s - 1) / in[v].s;
@@ -354,7 +405,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.lresf = lay.take(sizeof(float) * I.Lmax);
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
         o.numfx = lay.take(sizeof(uint64_t) * 2 * I.Lmax);
-        o.hist = lay.take(sizeof(uint64_t) * nb);
+        o.hist = lay.take(sizeof(uint64_t) * nb * hist_copies(I.W[0] * I.wtiles_x, tiled));
         o.spart = lay.take(sizeof(double) * 4 * I.W[0] * I.wtiles_x);
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
         o.etab = lay.take(sizeof(double) * nb);
@@ -417,6 +468,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.prep_tiles_x = I.prep_tiles_x;
         D.wtile_rows = I.wtile_rows;
         D.wtiles_x = I.wtiles_x;
+        D.hist_copies = hist_copies(I.W[0] * I.wtiles_x, tiled);
         D.mW2 = magic_div((uint32_t)I.W[2]);
         D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
@@ -631,14 +683,14 @@ static n4_status ensure_flags(n4_handle* h, int nvol) {
 
 // Non-blocking check of a group (volumes [v0, v1)) that has iterations
 // [0, it) of `level` enqueued on `stream`: *ready once every volume has
-// finished iteration it - kPipeline - 1 (its histogram pass it - kPipeline + 1
+// finished iteration it - depth - 1 (its histogram pass it - depth + 1
 // has run) or is done; *all_done once every volume is done.  Progress only
 // moves forward within a level, so a newer state than the one asked for is
 // harmless.  With `query`, a drained stream makes the words final (this
 // also ends the wait if a stream stopped early).
-static n4_status check_progress(n4_handle* h, int v0, int v1, int level, int it, void* stream, bool query,
-                                bool* ready, bool* all_done) {
-    const int need = it - kPipeline + 1;
+static n4_status check_progress(n4_handle* h, int v0, int v1, int level, int it, int depth, void* stream,
+                                bool query, bool* ready, bool* all_done) {
+    const int need = it - depth + 1;
     const volatile uint64_t* f = h->pinned_flags;
     bool rdy = true, all = true;
     for (int v = v0; v < v1; ++v) {
@@ -666,22 +718,63 @@ static n4_status check_progress(n4_handle* h, int v0, int v1, int level, int it,
     return N4_OK;
 }
 
-// Volume groups of a batch: each group runs its own chain of steps
-// on its own stream (group 0 on the caller's stream), so the small
-// per-volume kernels of one group overlap the slab kernels of the others and
-// a group whose volumes converge stops without waiting for the rest.
-// Volumes are dealt to groups largest first (working voxels), each to the
-// group with the least work so far.
-static int group_count(int nvol) {
-    int g = nvol >= 8 ? 4 : (nvol >= 4 ? 2 : 1);
-    if (const char* e = getenv("N4_GROUPS")) {
-        const int v = atoi(e);
-        if (v >= 1) g = v;
+// Identity of a captured step chunk: everything its launches read from the
+// host side (pointers, sizes, run parameters, chunk length).
+struct GraphKey {
+    std::vector<unsigned char> b;
+    template <class T>
+    void add(const T& v) {
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+        b.insert(b.end(), p, p + sizeof v);
     }
-    if (g > kMaxGroups) g = kMaxGroups;
-    if (g > nvol) g = nvol;
-    return g < 1 ? 1 : g;
+    void add_rp(const RunParams& r) {
+        add(r.twiddle);
+        add(r.nbins);
+        add(r.P);
+        add(r.log2P);
+        add(r.offset);
+        add(r.noise);
+        add(r.fwhm);
+        add(r.thr);
+        add(r.eps);
+        add(r.low);
+        add(r.low_f);
+        add(r.fixed_iters);
+        add(r.prec);
+    }
+};
+
+#ifndef N4_EMULATE
+// Launches group g's chunk graph on `stream`, capturing it first (on the
+// handle's capture stream, through `enqueue`) when the key changed.
+template <class F>
+static n4_status launch_step_graph(n4_handle* h, int g, const GraphKey& key, F&& enqueue, void* stream) {
+    n4_handle::StepGraph& sg = h->sgraph[g];
+    if (!sg.exec || sg.key != key.b) {
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+        sg.exec = nullptr;
+        sg.key.clear();
+        if (!h->cap_stream) N4_CHECK_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        N4_CHECK_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        const n4_status s = enqueue((void*)h->cap_stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(h->cap_stream, &graph);
+        if (s != N4_OK || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return s != N4_OK ? s : fail(N4_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        }
+        const hipError_t ei = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            sg.exec = nullptr;
+            return fail(N4_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        sg.key = key.b;
+    }
+    N4_CHECK_HIP(hipGraphLaunch(sg.exec, (hipStream_t)stream));
+    return N4_OK;
 }
+#endif
 
 struct GroupRun {
     int v0 = 0, v1 = 0;    // volumes
@@ -710,8 +803,10 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     }
     if (in_user.empty()) return N4_OK;
     const int nv = (int)in_user.size();
-    const int G = group_count(nv);
-    // order volumes by group (largest first onto the least loaded group)
+    const Schedule sch = resolve_schedule(h, nv);
+    const int G = sch.groups;
+    // order volumes by group: largest first (working voxels), each onto the
+    // group with the least work so far
     std::vector<int> order(nv), gid(nv, 0);
     {
         std::vector<std::pair<double, int>> sz(nv);
@@ -756,7 +851,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     N4_CHECK_HIP(hipEventRecord(ev.a, (hipStream_t)stream));
 #endif
     Plan P;
-    s = build_plan(h, in, p, &P, true, true);
+    s = build_plan(h, in, p, &P, true, true, sch.tile_vox);
     if (s != N4_OK) return s;
     s = check_lds(P, true, true);
     if (s != N4_OK) return s;
@@ -795,10 +890,20 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     }
     // Every volume runs its own level/iteration state machine on the device;
     // the host enqueues identical steps (hist -> wiener -> fit -> eval) per
-    // group, round-robin.  A group whose pipeline is full is skipped until
-    // its progress allows more, the host spins only when no group can
+    // group, round-robin, in chunks of `chunk` steps (one captured graph
+    // each, or direct launches).  A group whose pipeline is full is skipped
+    // until its progress allows more, the host spins only when no group can
     // advance, and a group ends once all its volumes are finished.  Every
-    // volume needs at most sum(max_iter + 1) + 1 steps.
+    // volume needs at most sum(max_iter + 1) + 1 steps; steps past a
+    // volume's end are skips.  Graphs are not used while kernels are timed
+    // with events.
+#ifndef N4_EMULATE
+    const bool use_graph = sch.graph_steps > 0 && h->prof_mask == 0;
+#else
+    const bool use_graph = false;
+#endif
+    const int chunk = sch.graph_steps > 0 ? sch.graph_steps : 1;
+    const int depth = chunk + kPipeline - 1;
     int max_steps = 1;
     for (int l = 0; l < P.nlevels; ++l) max_steps += P.max_iter[l] + 1;
     size_t lds_fit_all = 0, lds_eval_all = 0;
@@ -832,7 +937,8 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             bool end = R.it >= max_steps;
             if (R.it > 0 && !end) {
                 bool ready = false, all_done = false;
-                s = check_progress(h, R.v0, R.v1, 0, R.it, gs, (idle_rounds & 63) == 63, &ready, &all_done);
+                s = check_progress(h, R.v0, R.v1, 0, R.it, depth, gs, (idle_rounds & 63) == 63, &ready,
+                                   &all_done);
                 if (s != N4_OK) return s;
                 if (!ready) continue;
                 end = all_done;
@@ -842,11 +948,37 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             const VolDesc* gv = dd + R.v0;
             const Slab* gsl = wsl + R.ws0;
             if (!end) {
-                GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, R.it, rp, nullptr, P.lds_hist, gs));
-                GLAUNCH(N4_K_WIENER, launch_wiener(gv, nvg, rp, P.lds_wiener, gs));
-                GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, 0, 0, kFitStep, rp, lds_fit_all, mt, nt, gs));
-                GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, rp, lds_eval_all, mt, nt, gs));
-                ++R.it;
+                auto enqueue = [&](void* gs) -> n4_status {
+                    for (int k = 0; k < chunk; ++k) {
+                        GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, 0, rp, nullptr, P.lds_hist, gs));
+                        GLAUNCH(N4_K_WIENER, launch_wiener(gv, nvg, rp, P.lds_wiener, gs));
+                        GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, 0, 0, kFitStep, rp, lds_fit_all, mt, nt, gs));
+                        GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, rp, lds_eval_all, mt, nt, gs));
+                    }
+                    return N4_OK;
+                };
+                if (use_graph) {
+#ifndef N4_EMULATE
+                    GraphKey key;
+                    key.add(dd);
+                    key.add(gsl);
+                    key.add(nws);
+                    key.add(nvg);
+                    key.add(P.lds_hist);
+                    key.add(P.lds_wiener);
+                    key.add(lds_fit_all);
+                    key.add(lds_eval_all);
+                    key.add(mt);
+                    key.add(nt);
+                    key.add(chunk);
+                    key.add_rp(rp);
+                    s = launch_step_graph(h, g, key, enqueue, gs);
+#endif
+                } else {
+                    s = enqueue(gs);
+                }
+                if (s != N4_OK) return s;
+                R.it += chunk;
                 continue;
             }
             GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon, gs));
@@ -997,6 +1129,9 @@ n4_status n4_destroy(n4_handle* h) {
     }
     for (int g = 0; g < kMaxGroups; ++g)
         if (h->gstream[g]) (void)hipStreamDestroy(h->gstream[g]);
+    for (int g = 0; g < kMaxGroups; ++g)
+        if (h->sgraph[g].exec) (void)hipGraphExecDestroy(h->sgraph[g].exec);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
 #else
     free(h->pinned_flags);
 #endif
@@ -1029,6 +1164,67 @@ n4_status n4_correct_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, c
         in.push_back(v);
     }
     return run_batch(h, in, p, reports, stream);
+}
+
+n4_status n4_get_tuning(n4_handle* h, n4_tuning* out) {
+    if (!h || !out) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    *out = h->tune;
+    return N4_OK;
+}
+
+n4_status n4_set_tuning(n4_handle* h, const n4_tuning* t) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+    if (!t) {
+        h->tune = n4_tuning{};
+        return N4_OK;
+    }
+    if (t->groups > (uint32_t)kMaxGroups) return fail(N4_ERR_INVALID_ARG, "groups > %d", kMaxGroups);
+    if (t->step_tile_voxels != 0 && t->step_tile_voxels < 16)
+        return fail(N4_ERR_INVALID_ARG, "step_tile_voxels < 16");
+    if (t->graph_steps != N4_GRAPH_OFF && t->graph_steps > (uint32_t)kMaxGraphSteps)
+        return fail(N4_ERR_INVALID_ARG, "graph_steps > %d", kMaxGraphSteps);
+    h->tune = *t;
+    h->tune.ms = 0.0f;
+    return N4_OK;
+}
+
+// Exhaustive search over a small schedule grid, timed on the caller's own
+// volumes: one untimed run per candidate (captures its graphs), then the
+// best of two timed runs (host wall clock around the synchronous call).
+n4_status n4_autotune_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, const n4_params* p,
+                            n4_tuning* best, void* stream) {
+    if (!h || !p || !best) return fail(N4_ERR_INVALID_ARG, "NULL argument");
+    if (nvol == 0 || !vols) return fail(N4_ERR_INVALID_ARG, "no volumes to tune on");
+    const n4_tuning saved = h->tune;
+    std::vector<uint32_t> groups;
+    for (uint32_t g : {1u, 2u, 4u, 8u})
+        if (g <= nvol && g <= (uint32_t)kMaxGroups) groups.push_back(g);
+    const uint32_t tiles[] = {4096u, 8192u, 16384u};
+    const uint32_t graphs[] = {N4_GRAPH_OFF, 4u, 8u, 16u};
+    n4_tuning top{};
+    top.ms = -1.0f;
+    for (uint32_t g : groups)
+        for (uint32_t t : tiles)
+            for (uint32_t k : graphs) {
+                h->tune = n4_tuning{g, t, k, 0.0f};
+                double ms_best = -1.0;
+                for (int rep = 0; rep < 3; ++rep) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    const n4_status s = n4_correct_batch(h, vols, nvol, p, nullptr, stream);
+                    const auto t1 = std::chrono::steady_clock::now();
+                    if (s != N4_OK) {
+                        h->tune = saved;
+                        return s;
+                    }
+                    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+                    if (rep > 0 && (ms_best < 0.0 || ms < ms_best)) ms_best = ms;
+                }
+                if (top.ms < 0.0f || ms_best < (double)top.ms) top = n4_tuning{g, t, k, (float)ms_best};
+            }
+    h->tune = top;
+    h->tune.ms = 0.0f;
+    *best = top;
+    return N4_OK;
 }
 
 n4_status n4_correct_host(n4_handle* h, const float* h_im, const float* h_mask, uint32_t fx, uint32_t fy,

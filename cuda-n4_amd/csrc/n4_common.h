@@ -61,6 +61,7 @@ struct VolState {
     int32_t done;                     // all levels finished
     int32_t mode, lvl, it;            // state at the start of the next step
     int32_t s_kind, s_lvl, s_it;      // the current step (written by the histogram kernel)
+    int32_t passes;                   // histogram passes run (steps before the finish)
     uint64_t n_mask_full, n_mask_work;
     int32_t iters[kMaxLevels];
     double lastcv[kMaxLevels];
@@ -91,7 +92,7 @@ struct VolDesc {
     // partials
     float* part;               // per-slab fit contraction (DEN and FIELD modes)
     uint64_t* numfx[2];        // fixed-point lattice numerator, by iteration parity
-    uint64_t* hist;
+    uint64_t* hist;            // hist_copies x nbins fixed-point bins (summed by the Wiener step)
     double* spart;
     double* etab;
     VolState* st;
@@ -105,6 +106,7 @@ struct VolDesc {
     int32_t prep_tiles_x;      // prep tiles per working slab
     int32_t wtile_rows;        // working rows per tile of the step kernels
     int32_t wtiles_x;          // such tiles per working slab (spart has W0*wtiles_x entries)
+    int32_t hist_copies;       // tile t adds its histogram into copy t % hist_copies
     double* ppart;             // prep partials [W0 * prep_tiles_x][4]
     uint64_t mW2;              // magic divisor for the working z extent
     uint64_t mF2;              // magic divisor for the full-resolution z extent

@@ -41,8 +41,10 @@ int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink_hi
 // TRANS(0)); level > 0: refinement of level-1's lattice (stage call).
 int launch_level(const VolDesc* vols, int nvol, int level, RunParams rp, void* stream);
 // One correction step is hist -> wiener -> fit(kFitStep) -> eval; each
-// volume's step kind comes from its own state.  step: index of the step
-// (published as progress); step < 0 (stage calls) forces a histogram.
+// volume's step kind comes from its own state, and the launches of every
+// step are identical (a chunk of steps can be captured once and replayed).
+// step >= 0: a correction step (its pass count, kept in the state, is
+// published as progress); step < 0 (stage calls) forces a histogram.
 int launch_hist(const VolDesc* vols, const Slab* slabs, int nslab, int step, RunParams rp, int32_t* dbg_idx,
                 size_t lds, void* stream);
 int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void* stream);

@@ -421,7 +421,7 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
         V.numfx[0][l] = 0;
         V.numfx[1][l] = 0;
     }
-    for (int i = t; i < rp.nbins; i += kBlock) V.hist[i] = 0;
+    for (int i = t; i < rp.nbins * V.hist_copies; i += kBlock) V.hist[i] = 0;
     if (t == 0) {
         st->n_mask_full = (uint64_t)red[0];
         st->n_mask_work = (uint64_t)red[kBlock];
@@ -431,6 +431,7 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
         st->s_kind = kStepSkip;
         st->s_lvl = 0;
         st->s_it = 0;
+        st->passes = 0;
         st->done = 0;
         st->cv = 1e10;
         for (int l = 0; l < kMaxLevels; ++l) {
@@ -517,7 +518,10 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
             st->bin_max = mx;
             st->slope = slope;
             st->degenerate = degenerate ? 1 : 0;
-            if (step >= 0) publish_progress(V, 0, step + 1, kind == kStepFinal ? 1 : 0);
+            if (step >= 0) {
+                st->passes += 1;
+                publish_progress(V, 0, st->passes, kind == kStepFinal ? 1 : 0);
+            }
         }
         BLOCK_END
     }
@@ -563,9 +567,11 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     }
     BLOCK_END
     BLOCK_SYNC();
+    // tiles spread their flush over the copies: fewer atomics per address
+    uint64_t* dst = V.hist + (size_t)(tile_id(V, sl) % V.hist_copies) * nb;
     BLOCK_FOR(t)
     for (int i = t; i < nb; i += kBlock)
-        if (h[i]) global_add_u64(&V.hist[i], h[i]);
+        if (h[i]) global_add_u64(&dst[i], h[i]);
     BLOCK_END
 }
 
@@ -681,8 +687,13 @@ HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigne
         const double Fj = (j == 0) ? scale : scale * exp((double)(-(int64_t)jj * jj) * expF);
         double Vj = 0.0;
         if (j >= off && j < off + nb) {
-            Vj = (double)V.hist[j - off] * 0x1p-32;
-            V.hist[j - off] = 0;
+            uint64_t s = 0;  // exact: the copies are fixed-point
+            for (int k = 0; k < V.hist_copies; ++k) {
+                uint64_t* hp = V.hist + (size_t)k * nb + (j - off);
+                s += *hp;
+                *hp = 0;
+            }
+            Vj = (double)s * 0x1p-32;
         }
         z[j] = D2{Vj, Fj};
         tw[j] = rp.twiddle[j];
@@ -785,6 +796,8 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
 // reproducible): num[cx,p] = sum_x Px[x,cx] T2_x[p]  (N4.py:186-237).
 // The denominator of a level (mask contracted with the w^2 factors) is
 // accumulated the same way, into buffer 0, in the level's first step.
+// Zero terms (control points outside the tile's rows) add nothing and are
+// skipped.
 HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry& ex, int buf, bool q, int pr,
                           float t2) {
     const int CyCz = LD.C[1] * LD.C[2];
@@ -792,7 +805,7 @@ HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry
     UNROLL for (int k = 0; k < 4; ++k) {
         const double c = (double)(q ? ex.q[k] : ex.p[k]) * (double)t2;
         const long long q = (long long)rint(c * kNumScale);
-        global_add_u64(acc + (size_t)(ex.b + k) * CyCz + pr, (unsigned long long)q);
+        if (q != 0) global_add_u64(acc + (size_t)(ex.b + k) * CyCz + pr, (unsigned long long)q);
     }
 }
 

@@ -136,7 +136,14 @@ struct FitMfmaLds {
     Tab Pz;   // [NT*16][Zp + 8]  Pz^T: row = control point, column = z
     Tab Py;   // [MT*16][Yp + 4]  Py^T: row = control point, column = y
     float* red;
+    unsigned short* Gh;  // [4 waves][16][kGStride]  staged g, BF16 hi
+    unsigned short* Gl;  // same, lo (PREC 2)
 };
+
+// g staging: a wave loads 16 rows x 64 z with consecutive lanes on
+// consecutive z (256 B per load), remaps, and parks the BF16 halves here;
+// the MFMA A operands (8 consecutive z per lane) are then 16-B LDS reads.
+constexpr int kGStride = 72;  // halves per staged row (144 B: rows stay 16-B aligned)
 
 HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
     Carve c{base, 0};
@@ -150,6 +157,8 @@ HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, i
     L.Py.hi = c.take<unsigned short>((size_t)MT * 16 * ys);
     L.Py.lo = c.take<unsigned short>((size_t)MT * 16 * ys);
     L.red = c.take<float>((size_t)MT * NT * 256);
+    L.Gh = c.take<unsigned short>((size_t)4 * 16 * kGStride);
+    L.Gl = prec == 2 ? c.take<unsigned short>((size_t)4 * 16 * kGStride) : nullptr;
     if (o) *o = L;
     return align16(c.off);
 }
@@ -189,37 +198,50 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     f32x4 acc[MT][NT];
     UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0, 0, 0, 0};
 
-    for (int y0 = 16 * wv; y0 < Yt; y0 += 64) {
-        const bool yok = y0 + r16 < Yt;
-        const size_t row = ((size_t)x * Y + sl.y0 + (yok ? y0 + r16 : 0)) * Z;
+    unsigned short* Gh = S.Gh + wv * 16 * kGStride;
+    unsigned short* Gl = (PREC == 2) ? S.Gl + wv * 16 * kGStride : nullptr;
+    // every wave runs the same trip counts (barriers inside)
+    for (int rb = 0; rb < Yt; rb += 64) {
+        const int y0 = rb + 16 * wv;  // this wave's 16 tile-local rows (past Yt: zeros)
         f32x4 t1[NT];
         UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
-        for (int z0 = 0; z0 < Z; z0 += 32) {
-            const int zb = z0 + 8 * g4;
-            uint8_t ms[8];
-            float xs[8];
-            UNROLL for (int j = 0; j < 8; ++j) {
-                const bool ok = yok && (zb + j) < Z;
-                ms[j] = ok ? V.wmask[row + zb + j] : 0;
-                xs[j] = (ok && !den) ? V.logU[row + zb + j] : 0.0f;
+        for (int zc = 0; zc < Z; zc += 64) {
+            const int z = zc + lane;
+            uint8_t ms[16];
+            float xs[16];
+            UNROLL for (int r = 0; r < 16; ++r) {
+                const bool ok = z < Z && y0 + r < Yt;
+                const size_t v = ((size_t)x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
+                ms[r] = ok ? V.wmask[v] : 0;
+                xs[r] = (ok && !den) ? V.logU[v] : 0.0f;
             }
-            u16x8 ah, al;
-            UNROLL for (int j = 0; j < 8; ++j) {
-                const float gv = fit_input(gmode, ms[j], xs[j], degenerate, S.E, bmin, slope, nb);
+            UNROLL for (int r = 0; r < 16; ++r) {
+                const float gv = fit_input(gmode, ms[r], xs[r], degenerate, S.E, bmin, slope, nb);
                 unsigned short h, l = 0;
                 if constexpr (PREC == 2)
                     bf16_split(gv, h, l);
                 else
                     h = bf16_rn(gv);
-                ah[j] = h;
-                al[j] = l;
+                Gh[r * kGStride + lane] = h;
+                if constexpr (PREC == 2) Gl[r * kGStride + lane] = l;
             }
-            UNROLL for (int n = 0; n < NT; ++n) {
-                const u16x8 bh = S.Pz.get8(S.Pz.hi, n * 16 + r16, zb);
-                const u16x8 bl = S.Pz.get8(S.Pz.lo, n * 16 + r16, zb);
-                t1[n] = mma_k32p<PREC, true>(ah, al, bh, bl, t1[n]);
+            __syncthreads();
+            UNROLL for (int ks = 0; ks < 2; ++ks) {
+                const int z0 = zc + 32 * ks;
+                if (z0 < Z) {
+                    const u16x8 ah = *reinterpret_cast<const u16x8*>(Gh + r16 * kGStride + 32 * ks + 8 * g4);
+                    const u16x8 al =
+                        (PREC == 2) ? *reinterpret_cast<const u16x8*>(Gl + r16 * kGStride + 32 * ks + 8 * g4) : ah;
+                    UNROLL for (int n = 0; n < NT; ++n) {
+                        const u16x8 bh = S.Pz.get8(S.Pz.hi, n * 16 + r16, z0 + 8 * g4);
+                        const u16x8 bl = S.Pz.get8(S.Pz.lo, n * 16 + r16, z0 + 8 * g4);
+                        t1[n] = mma_k32p<PREC, true>(ah, al, bh, bl, t1[n]);
+                    }
+                }
             }
+            __syncthreads();
         }
+        if (y0 >= Yt) continue;  // no rows: nothing to add (and no Py rows)
         // t1[n]: lane holds T1[y0 + 4*g4 + i][16n + r16] = the K=16 operand (k = y)
         UNROLL for (int n = 0; n < NT; ++n) {
             i16x4 bh, bl;

@@ -35,7 +35,7 @@ EXPORTS = [
     "n4_convergence_measurement", "n4_upsample_lattice3d", "n4_reduce_sum", "n4_reduce_max",
     "n4_reduce_min", "n4_device_count", "n4_malloc", "n4_free", "n4_memcpy_htod",
     "n4_memcpy_dtoh", "n4_memcpy_dtod", "n4_memset", "n4_synchronize", "n4_profile_enable",
-    "n4_profile_read",
+    "n4_profile_read", "n4_get_tuning", "n4_set_tuning", "n4_autotune_batch",
 ]
 KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice"]
 
@@ -83,6 +83,24 @@ class Volume(C.Structure):
     ]
 
 
+GRAPH_OFF = 0xFFFFFFFF
+
+
+class Tuning(C.Structure):
+    """Launch schedule (include/n4_capi.h n4_tuning); 0 fields = defaults."""
+    _fields_ = [
+        ("groups", C.c_uint32),
+        ("step_tile_voxels", C.c_uint32),
+        ("graph_steps", C.c_uint32),
+        ("ms", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {"groups": int(self.groups), "step_tile_voxels": int(self.step_tile_voxels),
+                "graph_steps": "off" if self.graph_steps == GRAPH_OFF else int(self.graph_steps),
+                "ms": float(self.ms)}
+
+
 class N4Error(RuntimeError):
     pass
 
@@ -121,6 +139,9 @@ def _declare(L):
         "n4_synchronize": ([vp], C.c_int),
         "n4_profile_enable": ([vp, u32], C.c_int),
         "n4_profile_read": ([vp, u32, C.POINTER(dbl), C.POINTER(u64)], C.c_int),
+        "n4_get_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+        "n4_set_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+        "n4_autotune_batch": ([vp, C.POINTER(Volume), u32, C.POINTER(Params), C.POINTER(Tuning), vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -187,7 +208,10 @@ def default_params(**kw) -> Params:
         elif k == "fixed_iterations":
             p.flags = (p.flags | FLAG_FIXED_ITERATIONS) if v else (p.flags & ~FLAG_FIXED_ITERATIONS)
         else:
-            setattr(p, alias.get(k, k), v)
+            name = alias.get(k, k)
+            if name not in {f[0] for f in Params._fields_}:
+                raise TypeError(f"unknown parameter {k!r}")
+            setattr(p, name, v)
     return p
 
 
@@ -271,6 +295,31 @@ class Library:
         self._check(self.L.n4_profile_read(self.h, KERNELS.index(kernel), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    # ---------------------------------------------------- schedule ----
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(self.L.n4_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, groups=0, step_tile_voxels=0, graph_steps=0, reset=False):
+        """graph_steps: 0 default, "off" (or GRAPH_OFF) direct launches."""
+        if reset:
+            self._check(self.L.n4_set_tuning(self.h, None))
+            return
+        g = GRAPH_OFF if graph_steps == "off" else int(graph_steps)
+        t = Tuning(int(groups), int(step_tile_voxels), g, 0.0)
+        self._check(self.L.n4_set_tuning(self.h, C.byref(t)))
+
+    def autotune_batch(self, vols, params: Params | None = None, shrink=None, **kw) -> Tuning:
+        """Runs the library's schedule search on these host volumes and keeps
+        the winner on the handle."""
+        p = params if params is not None else default_params(**kw)
+        keep, arr = self._batch_arrays(vols, p, False, shrink)
+        best = Tuning()
+        self._check(self.L.n4_autotune_batch(self.h, arr, len(vols), C.byref(p), C.byref(best), None))
+        del keep
+        return best
+
     # ------------------------------------------------------------ N4 ----
     def correct(self, im, mask=None, params: Params | None = None, want_bias=False, want_log_bias=True,
                 mask_kind=None, **kw):
@@ -316,6 +365,21 @@ class Library:
         """vols: list of (im, mask) host arrays (mask may be None)."""
         p = params if params is not None else default_params(**kw)
         nc = final_ncpt(p)
+        keep, arr = self._batch_arrays(vols, p, want_log_bias, shrink)
+        reps = (Report * len(vols))()
+        self._check(self.L.n4_correct_batch(self.h, arr, len(vols), C.byref(p), reps, None))
+        res = []
+        for (shape, _, _, d_out, d_lat, d_lb), r in zip(keep, reps):
+            o = {"normalized": d_out.download(np.float32, shape), "lattice": d_lat.download(np.float32, nc),
+                 "iterations": [int(r.iterations[i]) for i in range(p.number_of_fitting_levels)],
+                 "n_mask_work": int(r.n_mask_work), "elapsed_ms": float(r.elapsed_ms)}
+            if d_lb is not None:
+                o["log_bias"] = d_lb.download(np.float32, shape)
+            res.append(o)
+        return res
+
+    def _batch_arrays(self, vols, p, want_log_bias, shrink):
+        nc = final_ncpt(p)
         keep, arr = [], (Volume * len(vols))()
         for i, (im, mask) in enumerate(vols):
             im = np.ascontiguousarray(im, dtype=np.float32)
@@ -336,17 +400,7 @@ class Library:
             v.d_lattice = d_lat.ptr
             v.d_bias_field = None
             v.d_log_bias_field = d_lb.ptr if d_lb else None
-        reps = (Report * len(vols))()
-        self._check(self.L.n4_correct_batch(self.h, arr, len(vols), C.byref(p), reps, None))
-        res = []
-        for (shape, _, _, d_out, d_lat, d_lb), r in zip(keep, reps):
-            o = {"normalized": d_out.download(np.float32, shape), "lattice": d_lat.download(np.float32, nc),
-                 "iterations": [int(r.iterations[i]) for i in range(p.number_of_fitting_levels)],
-                 "n_mask_work": int(r.n_mask_work), "elapsed_ms": float(r.elapsed_ms)}
-            if d_lb is not None:
-                o["log_bias"] = d_lb.download(np.float32, shape)
-            res.append(o)
-        return res
+        return keep, arr
 
     # -------------------------------------------------------- stages ----
     def histogram(self, x, mask, nbins=200, with_index=True):

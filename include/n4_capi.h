@@ -177,6 +177,33 @@ n4_status n4_reduce_sum(n4_handle *h, uint64_t n, const float *d_in, double *out
 n4_status n4_reduce_max(n4_handle *h, uint64_t n, const float *d_in, float *out);
 n4_status n4_reduce_min(n4_handle *h, uint64_t n, const float *d_in, float *out);
 
+/* ---- launch schedule and autotuner (no counterpart in the reference) --- */
+/* How n4_correct / n4_correct_batch lay out their launches.  Results do not
+ * depend on the schedule except through the summation order of the FP64
+ * convergence partials (one partial per tile).  A field left 0 takes the
+ * environment override (N4_GROUPS, N4_STEP_TILE_VOX, N4_GRAPH_STEPS) if set,
+ * else the built-in default. */
+#define N4_GRAPH_OFF 0xFFFFFFFFu
+typedef struct n4_tuning {
+    uint32_t groups;           /* volume groups, one stream each; default 4 for
+                                  >= 8 volumes, 2 for >= 4, else 1 (max 8) */
+    uint32_t step_tile_voxels; /* working voxels per workgroup tile of the
+                                  step kernels (rounded to whole rows); 8192 */
+    uint32_t graph_steps;      /* iteration steps per captured hipGraph,
+                                  1..64; N4_GRAPH_OFF = direct launches */
+    float ms;                  /* n4_autotune_batch: wall time of one batch */
+} n4_tuning;
+/* The handle's schedule as set (0 fields = defaults). */
+n4_status n4_get_tuning(n4_handle *h, n4_tuning *out);
+/* Sets the handle's schedule; NULL restores the defaults. */
+n4_status n4_set_tuning(n4_handle *h, const n4_tuning *t);
+/* Times n4_correct_batch on these volumes (outputs are written) under each
+ * candidate schedule -- groups {1,2,4,8} up to nvol, tiles {4096, 8192,
+ * 16384} voxels, graphs {off, 4, 8, 16} steps -- keeps the fastest on the
+ * handle and returns it in *best (ms = its best wall time). */
+n4_status n4_autotune_batch(n4_handle *h, const n4_volume *vols, uint32_t nvol,
+                            const n4_params *p, n4_tuning *best, void *stream);
+
 /* ---- kernel timing (no counterpart in the reference) ------------------- */
 /* Kernels of the correction, for n4_profile_*. */
 typedef enum n4_kernel_id {

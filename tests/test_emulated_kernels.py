@@ -69,3 +69,69 @@ def test_n4_row_tiles(emu, monkeypatch):
     monkeypatch.setenv("N4_STEP_TILE_VOX", "256")
     im, m, _ = PC.small_phantom((46, 38, 34), seed=9)
     PC.check_n4_vs_oracle(emu, im, m, 3, [8] * 3, 0.0, fixed=True)
+
+
+def _uneven_batch(seed0=40):
+    vols = []
+    for k, dims in enumerate([(30, 26, 22), (17, 33, 25), (41, 19, 28), (24, 24, 24), (33, 29, 18)]):
+        im, m, _ = PC.small_phantom(dims, seed=seed0 + k)
+        vols.append((im, m))
+    return vols
+
+
+def test_schedules_give_identical_results(emu):
+    """Groups and chunked step enqueueing change only the launch layout:
+    every such schedule returns the same fields bit for bit.  Row tiles also
+    regroup the FP32 partial sums of the fit before their fixed-point
+    accumulation, so a different tile size agrees to rounding only."""
+    vols = _uneven_batch()
+    kw = dict(levels=3, max_iter=[5, 5, 5], fixed_iterations=True)
+    try:
+        emu.set_tuning(reset=True)
+        ref = emu.correct_batch(vols, **kw)
+        for sched in [dict(groups=1, graph_steps="off"), dict(groups=2, graph_steps=3),
+                      dict(groups=5, step_tile_voxels=256, graph_steps=1), dict(groups=3, graph_steps=64)]:
+            emu.set_tuning(**sched)
+            out = emu.correct_batch(vols, **kw)
+            for a, b in zip(ref, out):
+                assert a["iterations"] == b["iterations"], sched
+                if "step_tile_voxels" in sched:
+                    assert np.max(np.abs(a["log_bias"] - b["log_bias"])) < 1e-5, sched
+                else:
+                    assert np.array_equal(a["log_bias"], b["log_bias"]), sched
+    finally:
+        emu.set_tuning(reset=True)
+
+
+def test_tuning_roundtrip_and_limits(emu):
+    import n4amd
+    try:
+        emu.set_tuning(groups=2, step_tile_voxels=4096, graph_steps="off")
+        t = emu.get_tuning().as_dict()
+        assert (t["groups"], t["step_tile_voxels"], t["graph_steps"]) == (2, 4096, "off")
+        for bad in [dict(groups=9), dict(step_tile_voxels=8), dict(graph_steps=65)]:
+            with pytest.raises(n4amd.N4Error):
+                emu.set_tuning(**bad)
+        emu.set_tuning(reset=True)
+        assert emu.get_tuning().as_dict()["groups"] == 0
+    finally:
+        emu.set_tuning(reset=True)
+
+
+def test_autotune_picks_a_candidate(emu):
+    """The search runs every candidate of its grid on the given volumes and
+    leaves the fastest on the handle."""
+    import n4amd
+    vols = _uneven_batch()[:3]
+    try:
+        best = emu.autotune_batch(vols, levels=2, max_iter=[2, 2])
+        d = best.as_dict()
+        assert d["groups"] in (1, 2) and d["step_tile_voxels"] in (4096, 8192, 16384)
+        assert d["graph_steps"] in ("off", 4, 8, 16) and d["ms"] > 0
+        kept = emu.get_tuning().as_dict()
+        assert {k: kept[k] for k in ("groups", "step_tile_voxels", "graph_steps")} == \
+            {k: d[k] for k in ("groups", "step_tile_voxels", "graph_steps")}
+        with pytest.raises(n4amd.N4Error):
+            emu.autotune_batch([], levels=2)
+    finally:
+        emu.set_tuning(reset=True)
