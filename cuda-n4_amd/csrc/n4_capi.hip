@@ -57,6 +57,7 @@ static void dev_free(void* p) {
 #define H2D(dst, src, n, st) N4_CHECK_HIP(hipMemcpyAsync((dst), (src), (n), hipMemcpyHostToDevice, (hipStream_t)(st)))
 #define D2H(dst, src, n, st) N4_CHECK_HIP(hipMemcpyAsync((dst), (src), (n), hipMemcpyDeviceToHost, (hipStream_t)(st)))
 #define D2D(dst, src, n, st) N4_CHECK_HIP(hipMemcpyAsync((dst), (src), (n), hipMemcpyDeviceToDevice, (hipStream_t)(st)))
+#define DSET(dst, v, n, st) N4_CHECK_HIP(hipMemsetAsync((dst), (v), (n), (hipStream_t)(st)))
 #define SYNC(st) N4_CHECK_HIP(hipStreamSynchronize((hipStream_t)(st)))
 #define LAUNCH(expr)                                                                         \
     do {                                                                                     \
@@ -74,6 +75,7 @@ static void dev_free(void* p) { free(p); }
 #define H2D(dst, src, n, st) memcpy((dst), (src), (n))
 #define D2H(dst, src, n, st) memcpy((dst), (src), (n))
 #define D2D(dst, src, n, st) memcpy((dst), (src), (n))
+#define DSET(dst, v, n, st) memset((dst), (v), (n))
 #define SYNC(st) \
     do {         \
     } while (0)
@@ -239,7 +241,7 @@ struct Plan {
     int n_wslab = 0, n_fslab = 0, n_pslab = 0;
     std::vector<int> max_iter;  // per level (max over volumes)
     std::vector<int> recon_tiles;  // recon tiles per volume
-    size_t lds_fit_max[kMaxLevels] = {0}, lds_eval_max[kMaxLevels] = {0};
+    size_t lds_fit_max[kMaxLevels] = {0}, lds_eval_max[kMaxLevels] = {0}, lds_eval32_max[kMaxLevels] = {0};
     int cx_max[kMaxLevels] = {0};
     int mt[kMaxLevels] = {0}, nt[kMaxLevels] = {0};  // MFMA tiles along y, z per level
     size_t lds_hist = 0, lds_wiener = 0, lds_recon = 0;
@@ -266,6 +268,7 @@ struct Schedule {
     int groups;       // volume groups (streams)
     int tile_vox;     // working voxels per step-kernel tile
     int graph_steps;  // steps per captured graph; 0 = direct launches
+    bool mfma_eval;   // MFMA field-increment kernel (MFMA precisions)
 };
 
 static int env_int(const char* name, int dflt) {
@@ -298,14 +301,23 @@ static Schedule resolve_schedule(const n4_handle* h, int nvol) {
     else
         k = env_int("N4_GRAPH_STEPS", kGraphSteps);
     s.graph_steps = std::min(std::max(k, 0), kMaxGraphSteps);
+    // the FP32 evaluation measured faster than the MFMA one on MI355X at
+    // every working-grid size tried, so it is the default in every mode
+    const uint32_t ek = h->tune.eval_kernel ? h->tune.eval_kernel
+                                            : (uint32_t)env_int("N4_EVAL_KERNEL", (int)N4_EVAL_FP32);
+    s.mfma_eval = ek == N4_EVAL_MFMA;
     return s;
 }
 
 // Copies of each volume's global histogram: the tiles of a correction
 // spread their atomic flushes over up to kHistCopies copies (stage calls
 // keep one, which they read back directly).
-static const int kHistCopies = 16;
+static const int kHistCopies = kMaxCopies;
 static int hist_copies(int tiles, bool tiled) { return tiled ? std::max(1, std::min(kHistCopies, tiles)) : 1; }
+// Copies of the fixed-point fit numerator, the same way (summed by the
+// lattice update of each step).
+static const int kNumCopies = kMaxCopies;
+static int num_copies(int tiles, bool tiled) { return tiled ? std::max(1, std::min(kNumCopies, tiles)) : 1; }
 
 static int rows_for(int rows, int cols, int target) {
     int r = target / (cols > 0 ? cols : 1);
@@ -396,15 +408,15 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         const size_t nw = (size_t)I.W[0] * I.W[1] * I.W[2];
         Offs& o = off[v];
         o.imlog = lay.take(sizeof(float) * nw);
-        o.logU = lay.take(sizeof(float) * nw);
-        o.wmask = lay.take(nw);
+        o.logU = lay.take(sizeof(float) * (nw + kQuadPad));
+        o.wmask = lay.take(nw + kQuadPad);
         o.lat0 = lay.take(sizeof(double) * I.Lmax);
         o.lat1 = lay.take(sizeof(double) * I.Lmax);
         o.den = lay.take(sizeof(double) * I.Lmax);
         o.latf = lay.take(sizeof(float) * I.Lmax);
         o.lresf = lay.take(sizeof(float) * I.Lmax);
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
-        o.numfx = lay.take(sizeof(uint64_t) * 2 * I.Lmax);
+        o.numfx = lay.take(sizeof(uint64_t) * I.Lmax * num_copies(I.W[0] * I.wtiles_x, tiled));
         o.hist = lay.take(sizeof(uint64_t) * nb * hist_copies(I.W[0] * I.wtiles_x, tiled));
         o.spart = lay.take(sizeof(double) * 4 * I.W[0] * I.wtiles_x);
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
@@ -459,8 +471,9 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.latf = (float*)(base + o.latf);
         D.lresf = (float*)(base + o.lresf);
         D.part = (float*)(base + o.part);
-        D.numfx[0] = (uint64_t*)(base + o.numfx);
-        D.numfx[1] = D.numfx[0] + I.Lmax;
+        D.numfx = (uint64_t*)(base + o.numfx);
+        D.num_copies = num_copies(I.W[0] * I.wtiles_x, tiled);
+        D.lmax = (int32_t)I.Lmax;
         D.hist = (uint64_t*)(base + o.hist);
         D.spart = (double*)(base + o.spart);
         D.ppart = (double*)(base + o.ppart);
@@ -514,6 +527,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         for (int l = 0; l < L; ++l) {
             P->lds_fit_max[l] = std::max(P->lds_fit_max[l], lds_fit(nb, D, l, prec));
             P->lds_eval_max[l] = std::max(P->lds_eval_max[l], lds_eval(D, l, prec));
+            P->lds_eval32_max[l] = std::max(P->lds_eval32_max[l], lds_eval(D, l, 0));
             P->cx_max[l] = std::max(P->cx_max[l], D.lev[l].C[0]);
             P->mt[l] = mfma_tiles(D.lev[l].C[1]);
             P->nt[l] = mfma_tiles(D.lev[l].C[2]);
@@ -909,9 +923,14 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     size_t lds_fit_all = 0, lds_eval_all = 0;
     for (int l = 0; l < P.nlevels; ++l) {
         lds_fit_all = std::max(lds_fit_all, P.lds_fit_max[l]);
-        lds_eval_all = std::max(lds_eval_all, P.lds_eval_max[l]);
+        lds_eval_all = std::max(lds_eval_all, sch.mfma_eval ? P.lds_eval_max[l] : P.lds_eval32_max[l]);
     }
     const int mt = P.mt[P.nlevels - 1], nt = P.nt[P.nlevels - 1];
+    RunParams rp_eval = rp;  // the evaluation kernel's precision
+    if (!sch.mfma_eval) rp_eval.prec = 0;
+    int lmax_all = 1;
+    for (int v = 0; v < nv; ++v) lmax_all = std::max(lmax_all, P.desc[v].lmax);
+    const int upd_chunks = (lmax_all + kBlock - 1) / kBlock;
 #ifndef N4_EMULATE
     if (G > 1) {
         N4_CHECK_HIP(hipEventRecord(h->fork_ev, (hipStream_t)stream));
@@ -953,7 +972,8 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                         GLAUNCH(N4_K_HIST, launch_hist(dd, gsl, nws, 0, rp, nullptr, P.lds_hist, gs));
                         GLAUNCH(N4_K_WIENER, launch_wiener(gv, nvg, rp, P.lds_wiener, gs));
                         GLAUNCH(N4_K_FIT, launch_fit(dd, gsl, nws, 0, 0, kFitStep, rp, lds_fit_all, mt, nt, gs));
-                        GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, rp, lds_eval_all, mt, nt, gs));
+                        GLAUNCH(N4_K_UPDATE, launch_update(gv, nvg, upd_chunks, rp, gs));
+                        GLAUNCH(N4_K_EVAL, launch_eval(dd, gsl, nws, rp_eval, lds_eval_all, mt, nt, gs));
                     }
                     return N4_OK;
                 };
@@ -971,6 +991,8 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                     key.add(mt);
                     key.add(nt);
                     key.add(chunk);
+                    key.add(upd_chunks);
+                    key.add(rp_eval.prec);
                     key.add_rp(rp);
                     s = launch_step_graph(h, g, key, enqueue, gs);
 #endif
@@ -1110,6 +1132,24 @@ n4_status n4_create(n4_handle** out, int device) {
 #endif
     n4_handle* h = new n4_handle();
     h->device = device;
+#ifndef N4_EMULATE
+    // HIP maps streams onto the process's few hardware queues in creation
+    // order: the group streams are created first, in group order, so that
+    // the first groups (with the null stream) own a queue each; the capture
+    // stream, which never executes work, comes last.
+    for (int g = 1; g < kMaxGroups; ++g) {
+        const hipError_t e = hipStreamCreateWithFlags(&h->gstream[g], hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            n4_destroy(h);
+            return fail(N4_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+        }
+    }
+    const hipError_t e = hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        n4_destroy(h);
+        return fail(N4_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    }
+#endif
     *out = h;
     return N4_OK;
 }
@@ -1183,6 +1223,7 @@ n4_status n4_set_tuning(n4_handle* h, const n4_tuning* t) {
         return fail(N4_ERR_INVALID_ARG, "step_tile_voxels < 16");
     if (t->graph_steps != N4_GRAPH_OFF && t->graph_steps > (uint32_t)kMaxGraphSteps)
         return fail(N4_ERR_INVALID_ARG, "graph_steps > %d", kMaxGraphSteps);
+    if (t->eval_kernel > N4_EVAL_MFMA) return fail(N4_ERR_INVALID_ARG, "unknown eval_kernel %u", t->eval_kernel);
     h->tune = *t;
     h->tune.ms = 0.0f;
     return N4_OK;
@@ -1196,31 +1237,56 @@ n4_status n4_autotune_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, 
     if (!h || !p || !best) return fail(N4_ERR_INVALID_ARG, "NULL argument");
     if (nvol == 0 || !vols) return fail(N4_ERR_INVALID_ARG, "no volumes to tune on");
     const n4_tuning saved = h->tune;
-    std::vector<uint32_t> groups;
-    for (uint32_t g : {1u, 2u, 4u, 8u})
-        if (g <= nvol && g <= (uint32_t)kMaxGroups) groups.push_back(g);
-    const uint32_t tiles[] = {4096u, 8192u, 16384u};
-    const uint32_t graphs[] = {N4_GRAPH_OFF, 4u, 8u, 16u};
-    n4_tuning top{};
-    top.ms = -1.0f;
-    for (uint32_t g : groups)
-        for (uint32_t t : tiles)
-            for (uint32_t k : graphs) {
-                h->tune = n4_tuning{g, t, k, 0.0f};
-                double ms_best = -1.0;
-                for (int rep = 0; rep < 3; ++rep) {
-                    const auto t0 = std::chrono::steady_clock::now();
-                    const n4_status s = n4_correct_batch(h, vols, nvol, p, nullptr, stream);
-                    const auto t1 = std::chrono::steady_clock::now();
-                    if (s != N4_OK) {
-                        h->tune = saved;
-                        return s;
-                    }
-                    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-                    if (rep > 0 && (ms_best < 0.0 || ms < ms_best)) ms_best = ms;
-                }
-                if (top.ms < 0.0f || ms_best < (double)top.ms) top = n4_tuning{g, t, k, (float)ms_best};
+    std::vector<n4_tuning> cand;
+    for (uint32_t g : {1u, 2u, 4u, 8u}) {
+        if (g > nvol || g > (uint32_t)kMaxGroups) continue;
+        for (uint32_t t : {4096u, 8192u, 16384u})
+            for (uint32_t k : {N4_GRAPH_OFF, 8u, 16u}) {
+                cand.push_back(n4_tuning{g, t, k, N4_EVAL_FP32, 0.0f});
+                if (p->precision != N4_PREC_FP32) cand.push_back(n4_tuning{g, t, k, N4_EVAL_MFMA, 0.0f});
             }
+    }
+    // best wall time of `reps` timed runs after one untimed run; N4_AUTOTUNE_LOG=1
+    // prints every measurement to stderr
+    const bool log = env_int("N4_AUTOTUNE_LOG", 0) != 0;
+    auto time_it = [&](n4_tuning& c, int reps) -> n4_status {
+        h->tune = c;
+        h->tune.ms = 0.0f;
+        double ms_best = -1.0;
+        for (int rep = 0; rep <= reps; ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const n4_status s = n4_correct_batch(h, vols, nvol, p, nullptr, stream);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (s != N4_OK) return s;
+            const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            if (rep > 0 && (ms_best < 0.0 || ms < ms_best)) ms_best = ms;
+        }
+        c.ms = (float)ms_best;
+        if (log)
+            fprintf(stderr, "[n4 autotune] groups %u tile %u graph %d eval %s: %.3f ms (best of %d)\n", c.groups,
+                    c.step_tile_voxels, c.graph_steps == N4_GRAPH_OFF ? 0 : (int)c.graph_steps,
+                    c.eval_kernel == N4_EVAL_MFMA ? "mfma" : "fp32", ms_best, reps);
+        return N4_OK;
+    };
+    for (n4_tuning& c : cand) {
+        const n4_status s = time_it(c, 2);
+        if (s != N4_OK) {
+            h->tune = saved;
+            return s;
+        }
+    }
+    // the three fastest again, with more runs, against timing noise
+    std::stable_sort(cand.begin(), cand.end(), [](const n4_tuning& a, const n4_tuning& b) { return a.ms < b.ms; });
+    const size_t nfin = std::min<size_t>(3, cand.size());
+    for (size_t i = 0; i < nfin; ++i) {
+        const n4_status s = time_it(cand[i], 5);
+        if (s != N4_OK) {
+            h->tune = saved;
+            return s;
+        }
+    }
+    const n4_tuning top = *std::min_element(cand.begin(), cand.begin() + nfin,
+                                            [](const n4_tuning& a, const n4_tuning& b) { return a.ms < b.ms; });
     h->tune = top;
     h->tune.ms = 0.0f;
     *best = top;
@@ -1263,7 +1329,11 @@ done:
 // ----------------------------------------------------------- stage calls --
 // Flat stage plans reuse the engine's kernels on a 1-D array cut into
 // kChunkVox-voxel slabs.
-static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const float* x, const uint8_t* m) {
+// hist: the histogram kernel will read x and m; it takes 16-byte loads of
+// aligned voxel quads, so the plan's own padded buffers get copies of them.
+// The flat kernels read x, m directly (m may be NULL: no mask).
+static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const float* x, const uint8_t* m,
+                           bool hist = false) {
     n4_params p;
     n4_params_default(&p);
     p.number_of_histogram_bins = (uint32_t)nbins;
@@ -1273,11 +1343,19 @@ static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const f
     n4_status s = build_plan(h, std::vector<VolIn>{v}, &p, P, false);
     if (s != N4_OK) return s;
     VolDesc& D = P->desc[0];
-    D.logU = const_cast<float*>(x);
-    D.wmask = const_cast<uint8_t*>(m);
     D.nvox_w = (int64_t)n;
+    if (!hist) {
+        D.logU = const_cast<float*>(x);
+        D.wmask = const_cast<uint8_t*>(m);
+    }
     memcpy(h->staging.data() + P->off_desc, &D, sizeof D);
     H2D(h->arena, h->staging.data(), P->upload_bytes, nullptr);
+    if (hist) {
+        const size_t nw = (size_t)nslab * kChunkVox;
+        D2D(D.logU, x, sizeof(float) * n, nullptr);
+        D2D(D.wmask, m, n, nullptr);
+        DSET(D.wmask + n, 0, nw + kQuadPad - n, nullptr);
+    }
     return N4_OK;
 }
 
@@ -1308,7 +1386,7 @@ n4_status n4_histogram(n4_handle* h, const float* d_x, const uint8_t* d_mask, ui
     if (!h || !d_x || !d_mask || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
     if (nbins < 2 || nbins > (uint32_t)kMaxBins) return fail(N4_ERR_INVALID_ARG, "nbins out of range");
     Plan P;
-    n4_status s = flat_plan(h, n, (int)nbins, &P, d_x, d_mask);
+    n4_status s = flat_plan(h, n, (int)nbins, &P, d_x, d_mask, true);
     if (s != N4_OK) return s;
     s = flat_hist(h, P, d_x, P.rp, d_bin_index);
     if (s != N4_OK) return s;
@@ -1337,7 +1415,7 @@ n4_status n4_sharpen_image(n4_handle* h, const float* d_before, const uint8_t* d
     if (nbins < 2 || nbins > (uint32_t)kMaxBins) return fail(N4_ERR_INVALID_ARG, "nbins out of range");
     if (!(fwhm > 0.0) || !(wiener_noise >= 0.0)) return fail(N4_ERR_INVALID_ARG, "bad Wiener parameters");
     Plan P;
-    n4_status s = flat_plan(h, n, (int)nbins, &P, d_before, d_mask);
+    n4_status s = flat_plan(h, n, (int)nbins, &P, d_before, d_mask, true);
     if (s != N4_OK) return s;
     RunParams rp = P.rp;
     rp.noise = wiener_noise;
@@ -1373,10 +1451,12 @@ n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_
     s = check_lds(P, true, false);
     if (s != N4_OK) return s;
     VolDesc& D = P.desc[0];
-    D.logU = const_cast<float*>(d_field);
-    D.wmask = const_cast<uint8_t*>(d_mask);
-    memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
     H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    // the fit kernel takes 16-byte loads of aligned voxel quads: it reads
+    // the plan's own padded copies of the field and the mask
+    const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    D2D(D.logU, d_field, sizeof(float) * nvox, nullptr);
+    D2D(D.wmask, d_mask, nvox, nullptr);
     VolDesc* dd = dev_desc(h, P);
     Slab* sl = dev_wslab(h, P);
     LAUNCH(launch_level(dd, 1, 0, P.rp, nullptr));

@@ -19,6 +19,14 @@
 namespace n4 {
 
 constexpr int kBlock = 256;
+constexpr int kHistLanes = 16;  // LDS histogram copies per workgroup (power of two)
+constexpr int kMaxCopies = 16;  // most global copies of a volume's histogram / fit numerator
+// logU and the working mask are allocated with kQuadPad extra elements, so
+// 16-byte (4-voxel) loads of aligned quads may run past a tile's end.
+constexpr int kQuadPad = 16;
+struct alignas(16) F4 {
+    float v[4];
+};
 constexpr int kMaxLevels = 16;
 constexpr int kMaxBins = 512;
 constexpr int kChunkVox = 4096;   // working voxels staged in LDS per row chunk
@@ -91,7 +99,7 @@ struct VolDesc {
     float* lresf;
     // partials
     float* part;               // per-slab fit contraction (DEN and FIELD modes)
-    uint64_t* numfx[2];        // fixed-point lattice numerator, by iteration parity
+    uint64_t* numfx;           // num_copies x lmax fixed-point lattice numerator copies
     uint64_t* hist;            // hist_copies x nbins fixed-point bins (summed by the Wiener step)
     double* spart;
     double* etab;
@@ -107,6 +115,8 @@ struct VolDesc {
     int32_t wtile_rows;        // working rows per tile of the step kernels
     int32_t wtiles_x;          // such tiles per working slab (spart has W0*wtiles_x entries)
     int32_t hist_copies;       // tile t adds its histogram into copy t % hist_copies
+    int32_t num_copies;        // tile t adds its fit numerator into copy t % num_copies
+    int32_t lmax;              // control points of the finest level (numfx copy stride)
     double* ppart;             // prep partials [W0 * prep_tiles_x][4]
     uint64_t mW2;              // magic divisor for the working z extent
     uint64_t mF2;              // magic divisor for the full-resolution z extent

@@ -54,6 +54,9 @@ int launch_wiener(const VolDesc* vols, int nvol, RunParams rp, size_t lds, void*
 int launch_fit(const VolDesc* vols, const Slab* slabs, int nslab, int level, int it, int mode, RunParams rp,
                size_t lds, int mt, int nt, void* stream);
 int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mode, RunParams rp, void* stream);
+// Lattice update of a step (numerator copies -> lattice increment or level
+// denominator); chunks = ceil(largest level size / kBlock).
+int launch_update(const VolDesc* vols, int nvol, int chunks, RunParams rp, void* stream);
 int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, size_t lds, int mt, int nt,
                 void* stream);
 // The final full-resolution reconstruction: FP32 in every precision mode.

@@ -417,10 +417,7 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
         V.latf[l] = 0.0f;
         V.lresf[l] = 0.0f;
     }
-    for (int l = t; l < Lmax; l += kBlock) {
-        V.numfx[0][l] = 0;
-        V.numfx[1][l] = 0;
-    }
+    for (int l = t; l < Lmax * V.num_copies; l += kBlock) V.numfx[l] = 0;
     for (int i = t; i < rp.nbins * V.hist_copies; i += kBlock) V.hist[i] = 0;
     if (t == 0) {
         st->n_mask_full = (uint64_t)red[0];
@@ -469,7 +466,10 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     const int nb = rp.nbins;
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
-    unsigned long long* h = cv.take<unsigned long long>(nb);
+    // kHistLanes lane-strided copies of the bins (odd stride: the copies of
+    // one bin fall in different LDS banks), merged before the global flush
+    const int hs = nb | 1;
+    unsigned long long* h = cv.take<unsigned long long>((size_t)kHistLanes * hs);
     const int mode = step < 0 ? kModeIter : st->mode;
     const int lvl = step < 0 ? 0 : st->lvl, it = step < 0 ? 0 : st->it;
     if (mode == kModeFinished) {
@@ -529,39 +529,48 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     const int YZ = sl.ny * V.W[2];
     const size_t base = ((size_t)sl.x * V.W[1] + sl.y0) * V.W[2];
     BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock) h[i] = 0ull;
+    for (int i = t; i < kHistLanes * hs; i += kBlock) h[i] = 0ull;
     BLOCK_END
     BLOCK_SYNC();
+    // aligned quads of voxels covering [base, vend): one 16-byte logU load
+    // and one 4-byte mask load per quad, four quads in flight per thread
+    const size_t vend = std::min(base + (size_t)YZ, (size_t)V.nvox_w);
+    const size_t q0 = base & ~(size_t)3;
+    const int nq = (int)((vend - q0 + 3) >> 2);
     BLOCK_FOR(t)
-    for (int v0 = 0; v0 < YZ; v0 += 4 * kBlock) {
-        float xs[4];
-        uint8_t ms[4];
+    unsigned long long* hl = h + (size_t)(t & (kHistLanes - 1)) * hs;
+    for (int i0 = 0; i0 < nq; i0 += 4 * kBlock) {
+        F4 xs[4];
+        uint32_t ms[4];
         UNROLL for (int u = 0; u < 4; ++u) {
-            const int v = v0 + u * kBlock + t;
-            const size_t w = base + v;
-            const bool ok = v < YZ && (int64_t)w < V.nvox_w;
-            ms[u] = ok ? V.wmask[w] : 0;
-            xs[u] = ok ? V.logU[w] : 0.0f;
+            const int qi = i0 + u * kBlock + t;
+            const size_t w = q0 + 4 * (size_t)qi;
+            const bool ok = qi < nq;
+            ms[u] = ok ? *reinterpret_cast<const uint32_t*>(V.wmask + w) : 0u;
+            xs[u] = ok ? *reinterpret_cast<const F4*>(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
         }
         UNROLL for (int u = 0; u < 4; ++u) {
-            const int v = v0 + u * kBlock + t;
-            const size_t w = base + v;
-            if (!ms[u]) {
-                if (dbg_idx && v < YZ && (int64_t)w < V.nvox_w) dbg_idx[w] = -1;
-                continue;
-            }
-            const double c = ((double)xs[u] - mn) / slope;
-            const double fl = floor(c);
-            const int idx = (fl >= 0.0 && fl < (double)nb) ? (int)fl : -1;
-            if (dbg_idx) dbg_idx[w] = idx;
-            if (idx < 0) continue;
-            const double off = c - fl;
-            if (off == 0.0) {
-                lds_add_u64(&h[idx], 1ull << 32);
-            } else if (idx < nb - 1) {
-                const unsigned long long q = (unsigned long long)rint(off * 4294967296.0);
-                lds_add_u64(&h[idx], (1ull << 32) - q);
-                lds_add_u64(&h[idx + 1], q);
+            const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
+            UNROLL for (int j = 0; j < 4; ++j) {
+                const size_t w = wq + j;
+                if (w < base || w >= vend) continue;
+                if (!((ms[u] >> (8 * j)) & 0xffu)) {
+                    if (dbg_idx) dbg_idx[w] = -1;
+                    continue;
+                }
+                const double c = ((double)xs[u].v[j] - mn) / slope;
+                const double fl = floor(c);
+                const int idx = (fl >= 0.0 && fl < (double)nb) ? (int)fl : -1;
+                if (dbg_idx) dbg_idx[w] = idx;
+                if (idx < 0) continue;
+                const double off = c - fl;
+                if (off == 0.0) {
+                    lds_add_u64(&hl[idx], 1ull << 32);
+                } else if (idx < nb - 1) {
+                    const unsigned long long q = (unsigned long long)rint(off * 4294967296.0);
+                    lds_add_u64(&hl[idx], (1ull << 32) - q);
+                    lds_add_u64(&hl[idx + 1], q);
+                }
             }
         }
     }
@@ -570,8 +579,11 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     // tiles spread their flush over the copies: fewer atomics per address
     uint64_t* dst = V.hist + (size_t)(tile_id(V, sl) % V.hist_copies) * nb;
     BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock)
-        if (h[i]) global_add_u64(&dst[i], h[i]);
+    for (int i = t; i < nb; i += kBlock) {
+        unsigned long long s = 0;
+        for (int c = 0; c < kHistLanes; ++c) s += h[(size_t)c * hs + i];
+        if (s) global_add_u64(&dst[i], s);
+    }
     BLOCK_END
 }
 
@@ -687,12 +699,15 @@ HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigne
         const double Fj = (j == 0) ? scale : scale * exp((double)(-(int64_t)jj * jj) * expF);
         double Vj = 0.0;
         if (j >= off && j < off + nb) {
-            uint64_t s = 0;  // exact: the copies are fixed-point
-            for (int k = 0; k < V.hist_copies; ++k) {
-                uint64_t* hp = V.hist + (size_t)k * nb + (j - off);
-                s += *hp;
-                *hp = 0;
-            }
+            // exact: the copies are fixed-point; all loads first, then the clears
+            uint64_t* hp = V.hist + (j - off);
+            const int nc = V.hist_copies;
+            uint64_t c[kMaxCopies];
+            UNROLL for (int k = 0; k < kMaxCopies; ++k) c[k] = k < nc ? hp[(size_t)k * nb] : 0;
+            uint64_t s = 0;
+            UNROLL for (int k = 0; k < kMaxCopies; ++k) s += c[k];
+            UNROLL for (int k = 0; k < kMaxCopies; ++k)
+                if (k < nc) hp[(size_t)k * nb] = 0;
             Vj = (double)s * 0x1p-32;
         }
         z[j] = D2{Vj, Fj};
@@ -745,8 +760,7 @@ HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigne
 // One block per volume: the step's per-volume work and the state for the
 // next step.
 //   ITER   Wiener step (unless degenerate); next: ITER(l, it+1)
-//   TRANS  refine the lattice into level l (l > 0), clear the numerator
-//          buffers; next: ITER(l, 0)
+//   TRANS  refine the lattice into level l (l > 0); next: ITER(l, 0)
 //   FINAL  FP32 copy of the final lattice; next: finished
 HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* smem) {
     const VolDesc& V = vols[bid];
@@ -764,10 +778,6 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
         const int l = st->s_lvl;
         BLOCK_FOR(t)
         if (l > 0) refine_lattice(V, l, t);
-        for (int i = t; i < V.lev[l].L; i += kBlock) {
-            V.numfx[0][i] = 0;
-            V.numfx[1][i] = 0;
-        }
         if (t == 0) {
             st->mode = kModeIter;
             st->lvl = l;
@@ -789,19 +799,19 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
 }
 
 // ---------------------------------------------------------- lattice update --
-// The fit of slab x adds its (y, z) contraction T2, weighted by the x
-// factor Px[x, cx], to the numerator of the 4 control planes cx = b(x) ..
-// b(x)+3 it touches, as fixed-point integer atomics into numfx[it & 1]
-// (integer sums do not depend on order, so the result is bitwise
-// reproducible): num[cx,p] = sum_x Px[x,cx] T2_x[p]  (N4.py:186-237).
-// The denominator of a level (mask contracted with the w^2 factors) is
-// accumulated the same way, into buffer 0, in the level's first step.
-// Zero terms (control points outside the tile's rows) add nothing and are
-// skipped.
-HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry& ex, int buf, bool q, int pr,
+// The fit of a tile of slab x adds its (y, z) contraction T2, weighted by
+// the x factor Px[x, cx], to the numerator of the 4 control planes cx =
+// b(x) .. b(x)+3 it touches, as fixed-point integer atomics into copy
+// tile % num_copies of the numerator (integer sums do not depend on order,
+// so the result is bitwise reproducible; the copies spread the atomics of
+// many tiles over more addresses): num[cx,p] = sum_x Px[x,cx] T2_x[p]
+// (N4.py:186-237).  The denominator of a level (mask contracted with the
+// w^2 factors) is accumulated the same way in the level's first step.
+// Zero terms (control points outside the tile's rows) are skipped.
+HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry& ex, int copy, bool q, int pr,
                           float t2) {
     const int CyCz = LD.C[1] * LD.C[2];
-    uint64_t* acc = V.numfx[buf];
+    uint64_t* acc = V.numfx + (size_t)copy * V.lmax;
     UNROLL for (int k = 0; k < 4; ++k) {
         const double c = (double)(q ? ex.q[k] : ex.p[k]) * (double)t2;
         const long long q = (long long)rint(c * kNumScale);
@@ -809,52 +819,43 @@ HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry
     }
 }
 
-// Lattice increment of the 4 control planes slab x touches: lres =
-// num / (den + eps) (N4.py:236).  The block of the first slab of a plane's
-// support also adds lres to the FP64 lattice (N4.py:317) and clears the
-// plane in the other numerator buffer for the next iteration, so each plane
-// is updated exactly once; planes without support have num = 0.  Runs
-// inside a thread phase (thread t of kBlock).
-HD void lattice_planes(const VolDesc& V, const LevelDesc& LD, double* lat, const Slab& sl, const AxisEntry& ex,
-                       int it, double eps, float* LR, int t) {
-    const int x = sl.x;
-    const int CyCz = LD.C[1] * LD.C[2];
-    const uint64_t* acc = V.numfx[it & 1];
-    uint64_t* next = V.numfx[(it + 1) & 1];
-    for (int k = 0; k < 4; ++k) {
-        const int cx = ex.b + k;
-        const bool owner = (x == LD.sup[0][2 * cx] && sl.y0 == 0);
-        for (int pr = t; pr < CyCz; pr += kBlock) {
-            const size_t l = (size_t)cx * CyCz + pr;
-            const double num = (double)(long long)acc[l] / kNumScale;
-            const double lres = num / (V.den[l] + eps);
-            if (owner) {
-                lat[l] += lres;
-                next[l] = 0;
-            }
-            LR[k * CyCz + pr] = (float)lres;
+// Lattice update between the fit and the evaluation of a step, one thread
+// per control point l of the volume's current level (block `chunk` of the
+// volume covers l = chunk*kBlock ..):
+//   ITER   lres = num / (den + eps) (N4.py:236); lat += lres (N4.py:317);
+//          lresf = lres in FP32 for the evaluation
+//   TRANS  den = num (the level's denominator)
+// num is the exact sum of the numerator copies, which are cleared for the
+// next fit.
+HD void body_update(const VolDesc* vols, RunParams rp, int chunk, int vol) {
+    const VolDesc& V = vols[vol];
+    const VolState* st = V.st;
+    const int kind = st->s_kind;
+    if (kind != kStepIter && kind != kStepTrans) return;
+    const int level = st->s_lvl;
+    const int L = V.lev[level].L;
+    BLOCK_FOR(t)
+    const int l = chunk * kBlock + t;
+    if (l < L) {
+        // all loads first (independent), then the clears
+        uint64_t* p = V.numfx + l;
+        const int nc = V.num_copies;
+        uint64_t c[kMaxCopies];
+        UNROLL for (int k = 0; k < kMaxCopies; ++k) c[k] = k < nc ? p[(size_t)k * V.lmax] : 0;
+        uint64_t sum = 0;
+        UNROLL for (int k = 0; k < kMaxCopies; ++k) sum += c[k];
+        UNROLL for (int k = 0; k < kMaxCopies; ++k)
+            if (k < nc) p[(size_t)k * V.lmax] = 0;
+        const double num = (double)(long long)sum / kNumScale;
+        if (kind == kStepTrans) {
+            V.den[l] = num;
+        } else {
+            const double lres = num / (V.den[l] + rp.eps);
+            V.lat[level & 1][l] += lres;
+            V.lresf[l] = (float)lres;
         }
     }
-}
-
-// First step of a level: the owner block of each of slab x's control planes
-// moves the fixed-point denominator into den (FP64) and clears buffer 0 for
-// the level's first numerator.  Runs inside a thread phase.
-HD void finalize_denominator(const VolDesc& V, int level, const Slab& sl, int t) {
-    const LevelDesc& LD = V.lev[level];
-    const int x = sl.x;
-    if (sl.y0 != 0) return;
-    const AxisEntry& ex = LD.ax[0][x];
-    const int CyCz = LD.C[1] * LD.C[2];
-    for (int k = 0; k < 4; ++k) {
-        const int cx = ex.b + k;
-        if (x != LD.sup[0][2 * cx]) continue;
-        for (int pr = t; pr < CyCz; pr += kBlock) {
-            const size_t l = (size_t)cx * CyCz + pr;
-            V.den[l] = (double)(long long)V.numfx[0][l] / kNumScale;
-            V.numfx[0][l] = 0;
-        }
-    }
+    BLOCK_END
 }
 
 // ------------------------------------------------------------------- fit --
@@ -976,19 +977,27 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
             for (int a = 0; a < 4; ++a) S.yw[4 * r + a] = den ? e.q[a] : e.p[a];
             S.yb[r] = e.b;
         }
-        for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
-            float xs[4];
-            uint8_t ms[4];
+        // aligned voxel quads covering the chunk (16-byte logU loads)
+        const size_t vend = base + (size_t)ny * Z, q0 = base & ~(size_t)3;
+        const int nq = (int)((vend - q0 + 3) >> 2);
+        for (int i0 = 0; i0 < nq; i0 += 4 * kBlock) {
+            F4 xs[4];
+            uint32_t ms[4];
             UNROLL for (int u = 0; u < 4; ++u) {
-                const int v = v0 + u * kBlock + t;
-                const bool ok = v < ny * Z;
-                ms[u] = ok ? V.wmask[base + v] : 0;
-                xs[u] = (ok && !den) ? V.logU[base + v] : 0.0f;
+                const int qi = i0 + u * kBlock + t;
+                const size_t w = q0 + 4 * (size_t)qi;
+                const bool ok = qi < nq;
+                ms[u] = ok ? *reinterpret_cast<const uint32_t*>(V.wmask + w) : 0u;
+                xs[u] = (ok && !den) ? *reinterpret_cast<const F4*>(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
             }
             UNROLL for (int u = 0; u < 4; ++u) {
-                const int v = v0 + u * kBlock + t;
-                if (v >= ny * Z) continue;
-                S.g[v] = fit_input(gmode, ms[u], xs[u], degenerate, S.E, bmin, slope, nb);
+                const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
+                UNROLL for (int j = 0; j < 4; ++j) {
+                    const size_t w = wq + j;
+                    if (w < base || w >= vend) continue;
+                    const uint8_t m = (uint8_t)((ms[u] >> (8 * j)) & 0xffu);
+                    S.g[w - base] = fit_input(gmode, m, xs[u].v[j], degenerate, S.E, bmin, slope, nb);
+                }
             }
         }
         BLOCK_END
@@ -1024,7 +1033,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     BLOCK_FOR(t)
     for (int pr = t; pr < Cy * Cz; pr += kBlock) {
         if (scatter)
-            scatter_numerator(V, LD, ex, num ? (it & 1) : 0, den, pr, S.T2[pr]);
+            scatter_numerator(V, LD, ex, tile_id(V, sl) % V.num_copies, den, pr, S.T2[pr]);
         else
             V.part[(size_t)x * V.part_stride + pr] = S.T2[pr];
     }
@@ -1101,7 +1110,6 @@ HD void body_lattice(const VolDesc* vols, int level, int mode, RunParams rp, int
 //   logU -= d  (= imlog - new field);  u = exp(old-new) - 1 = expm1(-d)
 // Per slab: spart[x] = (sum u, sum u^2 in FP64, min, max of logU).
 struct EvalLds {
-    float* LR;
     float* Ar;
     float* Br;
     float* zw;
@@ -1112,7 +1120,6 @@ struct EvalLds {
 HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds* o) {
     Carve c{base, 0};
     EvalLds L;
-    L.LR = c.take<float>((size_t)4 * Cy * Cz);
     L.Ar = c.take<float>((size_t)Cy * Cz);
     L.Br = c.take<float>((size_t)R * Cz);
     L.zw = c.take<float>((size_t)4 * Z);
@@ -1125,15 +1132,8 @@ HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds*
 HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    const int kind = V.st->s_kind;
-    if (kind == kStepTrans) {
-        BLOCK_FOR(t)
-        finalize_denominator(V, V.st->s_lvl, sl, t);
-        BLOCK_END
-        return;
-    }
-    if (kind != kStepIter) return;
-    const int level = V.st->s_lvl, it = V.st->s_it;
+    if (V.st->s_kind != kStepIter) return;
+    const int level = V.st->s_lvl;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
@@ -1143,14 +1143,11 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     carve_eval(smem, Z, R, Cy, Cz, &S);
     const int x = sl.x;
     const AxisEntry ex = LD.ax[0][x];
-    BLOCK_FOR(t)
-    lattice_planes(V, LD, V.lat[level & 1], sl, ex, it, rp.eps, S.LR, t);
-    BLOCK_END
-    BLOCK_SYNC();
+    const float* LR = V.lresf + (size_t)ex.b * CyCz;  // the 4 increment planes slab x touches
     BLOCK_FOR(t)
     for (int pr = t; pr < CyCz; pr += kBlock) {
         float ar = 0.0f;
-        for (int k = 0; k < 4; ++k) ar += ex.w[k] * S.LR[k * CyCz + pr];
+        for (int k = 0; k < 4; ++k) ar += ex.w[k] * LR[k * CyCz + pr];
         S.Ar[pr] = ar;
     }
     for (int z = t; z < Z; z += kBlock) {
@@ -1181,16 +1178,18 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
         BLOCK_FOR(t)
         double s1 = S.red[t], s2 = S.red[kBlock + t];
         double mn = S.red[2 * kBlock + t], mx = S.red[3 * kBlock + t];
-        for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
-            uint8_t ms[4];
-            float lu[4];
-            UNROLL for (int u = 0; u < 4; ++u) {
+        // consecutive lanes on consecutive voxels (conflict-free LDS
+        // reads of the z tables), eight loads in flight per thread
+        for (int v0 = 0; v0 < ny * Z; v0 += 8 * kBlock) {
+            uint8_t ms[8];
+            float lu[8];
+            UNROLL for (int u = 0; u < 8; ++u) {
                 const int v = v0 + u * kBlock + t;
                 const bool ok = v < ny * Z;
                 ms[u] = ok ? V.wmask[base + v] : 0;
                 lu[u] = (ok && ms[u]) ? V.logU[base + v] : 0.0f;
             }
-            UNROLL for (int u = 0; u < 4; ++u) {
+            UNROLL for (int u = 0; u < 8; ++u) {
                 if (!ms[u]) continue;
                 const int v = v0 + u * kBlock + t;
                 const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
@@ -1422,7 +1421,9 @@ static int cp_tiles(int c) { return (c + 15) / 16; }
 
 size_t lds_prep() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_level() { return align16(4 * kBlock * sizeof(double)); }
-size_t lds_hist(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(uint64_t)); }
+size_t lds_hist(int nbins) {
+    return align16(4 * kBlock * sizeof(double)) + align16((size_t)kHistLanes * (nbins | 1) * sizeof(uint64_t));
+}
 size_t lds_wiener(int P) { return align16(4 * (size_t)P * sizeof(D2)); }
 size_t lds_lattice() { return align16(kBlock * sizeof(double)); }
 size_t lds_flat(int nbins) { return align16(4 * kBlock * sizeof(double)) + align16(nbins * sizeof(double)); }
@@ -1493,6 +1494,9 @@ __global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* vols, int lev
     body_lattice(vols, level, mode, rp, blockIdx.x, blockIdx.y, SMEM());
 }
 
+__global__ __launch_bounds__(kBlock) void k_update(const VolDesc* vols, RunParams rp) {
+    body_update(vols, rp, (int)blockIdx.x, (int)blockIdx.y);
+}
 __global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_eval(vols, slabs, rp, blockIdx.x, SMEM());
@@ -1598,6 +1602,12 @@ int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mod
     return last_launch_status();
 }
 
+int launch_update(const VolDesc* vols, int nvol, int chunks, RunParams rp, void* stream) {
+    if (nvol <= 0 || chunks <= 0) return 0;
+    hipLaunchKernelGGL(k_update, dim3(chunks, nvol), dim3(kBlock), 0, (hipStream_t)stream, vols, rp);
+    return last_launch_status();
+}
+
 int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp, size_t lds, int mt, int nt,
                 void* stream) {
     if (nslab <= 0) return 0;
@@ -1670,6 +1680,12 @@ int launch_lattice(const VolDesc* vols, int nvol, int cx_max, int level, int mod
     unsigned char* b = emu_lds(lds_lattice());
     for (int v = 0; v < nvol; ++v)
         for (int cx = 0; cx < cx_max; ++cx) body_lattice(vols, level, mode, rp, cx, v, b);
+    return 0;
+}
+
+int launch_update(const VolDesc* vols, int nvol, int chunks, RunParams rp, void*) {
+    for (int v = 0; v < nvol; ++v)
+        for (int c = 0; c < chunks; ++c) body_update(vols, rp, c, v);
     return 0;
 }
 

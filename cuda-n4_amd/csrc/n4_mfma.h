@@ -271,7 +271,7 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
         const int cy = m * 16 + 4 * (ln >> 4) + i, cz = n * 16 + (ln & 15);
         if (cy < Cy && cz < Cz) {
             if (scatter)
-                scatter_numerator(V, LD, ex, num ? (it & 1) : 0, den, cy * Cz + cz, S.red[e]);
+                scatter_numerator(V, LD, ex, tile_id(V, sl) % V.num_copies, den, cy * Cz + cz, S.red[e]);
             else
                 out[cy * Cz + cz] = S.red[e];
         }
@@ -286,7 +286,6 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
 // HBM and stays on the FP32 kernel in every mode: an MFMA form with its
 // BF16 tables in LDS ran 5x slower there, limited by occupancy.)
 struct EvalMfmaLds {
-    float* LR;   // [4][Cy*Cz]  lattice increment planes
     float* A;    // [Cy*Cz]     x-contracted planes
     Tab At;      // [NT*16][MT*16 + 4]  A^T: row = cz, column = cy
     Tab Wy;      // [Yrows][MT*16 + 4]  row = y (slab-local), column = cy
@@ -298,7 +297,6 @@ HD size_t carve_eval_mfma(unsigned char* base, int Yrows, int Z, int Cy, int Cz,
                           EvalMfmaLds* o) {
     Carve c{base, 0};
     EvalMfmaLds L;
-    L.LR = c.take<float>((size_t)4 * Cy * Cz);
     L.A = c.take<float>((size_t)Cy * Cz);
     const int as = MT * 16 + 4, zs = NT * 16 + 4;
     L.At.stride = as;
@@ -378,13 +376,8 @@ template <int PREC, int MT, int NT>
 __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    const int kind = V.st->s_kind;
-    if (kind == kStepTrans) {
-        finalize_denominator(V, V.st->s_lvl, sl, (int)threadIdx.x);
-        return;
-    }
-    if (kind != kStepIter) return;
-    const int level = V.st->s_lvl, it = V.st->s_it;
+    if (V.st->s_kind != kStepIter) return;
+    const int level = V.st->s_lvl;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
@@ -393,10 +386,9 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
     const int x = sl.x;
     const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
     const AxisEntry ex = LD.ax[0][x];
-    lattice_planes(V, LD, V.lat[level & 1], sl, ex, it, rp.eps, S.LR, t);
-    __syncthreads();
     const int Yt = sl.ny;  // the tile's rows (Wy rows are tile-local)
-    eval_mfma_setup<MT, NT>(S, ex, S.LR, LD.ax[1], sl.y0, Yt, LD.ax[2], Z, Cy, Cz);
+    // the 4 increment planes slab x touches (FP32, written by the update kernel)
+    eval_mfma_setup<MT, NT>(S, ex, V.lresf + (size_t)ex.b * Cy * Cz, LD.ax[1], sl.y0, Yt, LD.ax[2], Z, Cy, Cz);
     double s1 = 0.0, s2 = 0.0, mn = INFINITY, mx = -INFINITY;
     for (int y0 = 16 * wv; y0 < Yt; y0 += 64) {
         i16x4 bh[NT], bl[NT];

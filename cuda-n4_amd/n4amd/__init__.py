@@ -37,7 +37,7 @@ EXPORTS = [
     "n4_memcpy_dtoh", "n4_memcpy_dtod", "n4_memset", "n4_synchronize", "n4_profile_enable",
     "n4_profile_read", "n4_get_tuning", "n4_set_tuning", "n4_autotune_batch",
 ]
-KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice"]
+KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice", "update"]
 
 
 class Params(C.Structure):
@@ -84,6 +84,8 @@ class Volume(C.Structure):
 
 
 GRAPH_OFF = 0xFFFFFFFF
+EVAL_KERNELS = {"fp32": 1, "mfma": 2}
+EVAL_KERNELS_INV = {v: k for k, v in EVAL_KERNELS.items()}
 
 
 class Tuning(C.Structure):
@@ -92,13 +94,14 @@ class Tuning(C.Structure):
         ("groups", C.c_uint32),
         ("step_tile_voxels", C.c_uint32),
         ("graph_steps", C.c_uint32),
+        ("eval_kernel", C.c_uint32),
         ("ms", C.c_float),
     ]
 
     def as_dict(self):
         return {"groups": int(self.groups), "step_tile_voxels": int(self.step_tile_voxels),
                 "graph_steps": "off" if self.graph_steps == GRAPH_OFF else int(self.graph_steps),
-                "ms": float(self.ms)}
+                "eval_kernel": EVAL_KERNELS_INV.get(int(self.eval_kernel), 0), "ms": float(self.ms)}
 
 
 class N4Error(RuntimeError):
@@ -301,13 +304,15 @@ class Library:
         self._check(self.L.n4_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, groups=0, step_tile_voxels=0, graph_steps=0, reset=False):
-        """graph_steps: 0 default, "off" (or GRAPH_OFF) direct launches."""
+    def set_tuning(self, groups=0, step_tile_voxels=0, graph_steps=0, eval_kernel=0, reset=False):
+        """graph_steps: 0 default, "off" (or GRAPH_OFF) direct launches;
+        eval_kernel: 0 default, "fp32" or "mfma"."""
         if reset:
             self._check(self.L.n4_set_tuning(self.h, None))
             return
         g = GRAPH_OFF if graph_steps == "off" else int(graph_steps)
-        t = Tuning(int(groups), int(step_tile_voxels), g, 0.0)
+        e = EVAL_KERNELS[eval_kernel] if isinstance(eval_kernel, str) else int(eval_kernel)
+        t = Tuning(int(groups), int(step_tile_voxels), g, e, 0.0)
         self._check(self.L.n4_set_tuning(self.h, C.byref(t)))
 
     def autotune_batch(self, vols, params: Params | None = None, shrink=None, **kw) -> Tuning:

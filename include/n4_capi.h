@@ -191,16 +191,22 @@ typedef struct n4_tuning {
                                   step kernels (rounded to whole rows); 8192 */
     uint32_t graph_steps;      /* iteration steps per captured hipGraph,
                                   1..64; N4_GRAPH_OFF = direct launches */
+    uint32_t eval_kernel;      /* field-increment kernel of the MFMA
+                                  precisions: N4_EVAL_FP32 (default) or
+                                  N4_EVAL_MFMA; FP32 runs use FP32 */
     float ms;                  /* n4_autotune_batch: wall time of one batch */
 } n4_tuning;
+#define N4_EVAL_FP32 1u
+#define N4_EVAL_MFMA 2u
 /* The handle's schedule as set (0 fields = defaults). */
 n4_status n4_get_tuning(n4_handle *h, n4_tuning *out);
 /* Sets the handle's schedule; NULL restores the defaults. */
 n4_status n4_set_tuning(n4_handle *h, const n4_tuning *t);
 /* Times n4_correct_batch on these volumes (outputs are written) under each
  * candidate schedule -- groups {1,2,4,8} up to nvol, tiles {4096, 8192,
- * 16384} voxels, graphs {off, 4, 8, 16} steps -- keeps the fastest on the
- * handle and returns it in *best (ms = its best wall time). */
+ * 16384} voxels, graphs {off, 8, 16} steps and, for the MFMA precisions,
+ * both field-increment kernels -- keeps the fastest on the handle and
+ * returns it in *best (ms = its best wall time). */
 n4_status n4_autotune_batch(n4_handle *h, const n4_volume *vols, uint32_t nvol,
                             const n4_params *p, n4_tuning *best, void *stream);
 
@@ -215,8 +221,9 @@ typedef enum n4_kernel_id {
     N4_K_EVAL = 5,    /* field + convergence partials + logU update */
     N4_K_RECON = 6,   /* full-resolution field + normalisation */
     N4_K_WIENER = 7,  /* Wiener deconvolution + E table (one block/volume) */
-    N4_K_LATTICE = 8, /* mode-x contraction + lattice update */
-    N4_K_COUNT = 9
+    N4_K_LATTICE = 8, /* mode-x contraction + lattice update (stage calls) */
+    N4_K_UPDATE = 9,  /* lattice update of a correction step */
+    N4_K_COUNT = 10
 } n4_kernel_id;
 /* Time every launch of the kernels in kernel_mask (bit = n4_kernel_id) with
  * HIP events on the launch stream; clears the accumulators. 0 disables. */

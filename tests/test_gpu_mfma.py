@@ -1,7 +1,9 @@
 """BF16 and split-BF16 MFMA forms of the separable B-spline contractions
-(fit numerator and denominator, working-grid evaluation of the field
-increments) against the FP64 oracle.  The final full-resolution
-reconstruction is the FP32 kernel in every mode.
+(fit numerator and denominator, and -- when selected -- the working-grid
+evaluation of the field increments) against the FP64 oracle.  The final
+full-resolution reconstruction is the FP32 kernel in every mode; the
+per-iteration evaluation is the FP32 kernel by default and the MFMA one
+with eval_kernel="mfma".
 
 Tolerances on the log bias field (max abs over masked voxels):
   * split-BF16 (voxel operands and basis factors as hi + lo BF16 halves,
@@ -23,10 +25,17 @@ pytestmark = pytest.mark.gpu
 TOL = {"split_bf16": 1e-4, "bf16": 1e-3}
 
 
+@pytest.fixture(params=["fp32", "mfma"])
+def eval_kernel(gpu, request):
+    gpu.set_tuning(eval_kernel=request.param)
+    yield request.param
+    gpu.set_tuning(reset=True)
+
+
 @pytest.mark.parametrize("precision", ["split_bf16", "bf16"])
-def test_fixed_iterations_vs_oracle(gpu, precision):
+def test_fixed_iterations_vs_oracle(gpu, precision, eval_kernel):
     im, m, _ = PC.small_phantom((48, 40, 36), seed=2)
-    print(f"{precision}: 48x40x36, 3 levels x 10 fixed iterations", flush=True)
+    print(f"{precision} ({eval_kernel} eval): 48x40x36, 3 levels x 10 fixed iterations", flush=True)
     e, rms, _ = PC.check_n4_vs_oracle(gpu, im, m, 3, [10] * 3, 0.0, fixed=True, precision=precision,
                                       tol=TOL[precision])
     print(f"{precision}: max err {e:.3g} rms {rms:.3g}")
@@ -50,20 +59,20 @@ def test_split_golden_trajectory(gpu, golden):
 
 
 @pytest.mark.parametrize("shrink", [2, 3])
-def test_split_shrink_partial_tiles(gpu, shrink):
+def test_split_shrink_partial_tiles(gpu, shrink, eval_kernel):
     """Edge tiles everywhere: 75x62x54 leaves partial 16-row tiles and
     partial 16- and 32-wide z tiles on the working grid."""
     im, m, _ = PC.small_phantom((75, 62, 54), seed=9)
-    print(f"split_bf16: shrink {shrink} on 75x62x54", flush=True)
+    print(f"split_bf16 ({eval_kernel} eval): shrink {shrink} on 75x62x54", flush=True)
     PC.check_n4_vs_oracle(gpu, im, m, 3, [12] * 3, 0.0, shrink=shrink, fixed=True, precision="split_bf16")
 
 
 @pytest.mark.parametrize("precision", ["split_bf16", "bf16"])
-def test_multi_tile_lattice(gpu, precision):
+def test_multi_tile_lattice(gpu, precision, eval_kernel):
     """ncpt (4, 5, 6) over 4 levels ends at 11 x 19 x 27 control points: two
     16-point MFMA tiles along y and z."""
     im, m, _ = PC.small_phantom((40, 56, 72), seed=4)
-    print(f"{precision}: ncpt (4,5,6), 4 levels", flush=True)
+    print(f"{precision} ({eval_kernel} eval): ncpt (4,5,6), 4 levels", flush=True)
     e, rms, _ = PC.check_n4_vs_oracle(gpu, im, m, 4, [3] * 4, 0.0, ncpt=(4, 5, 6), fixed=True,
                                       precision=precision, tol=TOL[precision])
     print(f"{precision}: multi-tile max err {e:.3g} rms {rms:.3g}")

@@ -81,7 +81,7 @@ def test_autotune_batch(gpu):
         best = gpu.autotune_batch(vols, **kw).as_dict()
         print(f"autotuned: {best}", flush=True)
         assert best["groups"] in (1, 2, 4) and best["step_tile_voxels"] in (4096, 8192, 16384)
-        assert best["graph_steps"] in ("off", 4, 8, 16) and best["ms"] > 0
+        assert best["graph_steps"] in ("off", 8, 16) and best["ms"] > 0
         out = gpu.correct_batch(vols, **kw)
         for a, b in zip(ref, out):
             assert a["iterations"] == b["iterations"]
