@@ -221,31 +221,37 @@ def main():
             lib.set_tuning(graph_steps="off")
         schedule = lib.get_tuning().as_dict()
         schedule.pop("ms", None)
-        # warmup with every kernel timed: picks the dominant kernel
-        lib.profile_enable(n4amd.KERNELS)
-        for _ in range(max(1, args.warmup)):
-            batch_step(p)
-        breakdown = {k: lib.profile_read(k) for k in n4amd.KERNELS}
-        dominant = max(breakdown, key=lambda k: breakdown[k][0])
-        log(f"[{prec}] warmup per-kernel device time (ms, launches): "
-            + ", ".join(f"{k}={v[0] / max(1, args.warmup):.3f}/{v[1] // max(1, args.warmup)}"
-                        for k, v in breakdown.items()))
-        # timed region: no events in the stream
+        # untimed warmup and the timed region, with the batch's schedule and
+        # no events in the streams
         lib.profile_enable([])
+        for _ in range(args.warmup):
+            batch_step(p)
         lib.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             batch_step(p)
         lib.synchronize()
         dt = time.perf_counter() - t0
-        # the same number of steps again with events around every launch of
-        # the dominant kernel (on its own stream): its average duration
+        # kernel durations, measured apart from the timed region with one
+        # stream group and direct launches (no concurrent groups sharing the
+        # CUs, no graphs): one step with events around every launch picks
+        # the dominant kernel, then the same number of steps as timed with
+        # events around each of its launches give its average duration
+        lib.set_tuning(**dict(schedule, groups=1, graph_steps="off"))
+        lib.profile_enable(n4amd.KERNELS)
+        batch_step(p)
+        lib.synchronize()
+        breakdown = {k: lib.profile_read(k) for k in n4amd.KERNELS}
+        dominant = max(breakdown, key=lambda k: breakdown[k][0])
+        log(f"[{prec}] one-group per-kernel device time (ms, launches): "
+            + ", ".join(f"{k}={v[0]:.3f}/{v[1]}" for k, v in breakdown.items()))
         lib.profile_enable([dominant])
         for _ in range(args.steps):
             batch_step(p)
         lib.synchronize()
         dom_ms, dom_n = lib.profile_read(dominant)
         lib.profile_enable([])
+        lib.set_tuning(**schedule)
         its = [[int(r.iterations[l]) for l in range(cfg["levels"])] for r in reps]
         vox_iter = sum(m["n_work"] * sum(i) for m, i in zip(meta, its))
         dom_bytes = algorithmic_bytes(dominant, meta, reps, cfg["levels"]) * args.steps
@@ -255,7 +261,7 @@ def main():
              "iterations_per_level_mean": [round(float(np.mean([i[l] for i in its])), 2)
                                            for l in range(cfg["levels"])],
              "dominant": dominant, "dom_ms": dom_ms, "dom_n": dom_n, "dom_bytes": dom_bytes, "achieved": achieved,
-             "breakdown": {k: round(v[0] / max(1, args.warmup), 4) for k, v in breakdown.items()},
+             "breakdown": {k: round(v[0], 4) for k, v in breakdown.items()},
              "err": sample_errors(its), "schedule": schedule, "autotune_s": round(t_tune, 2)}
         runs[prec] = R
         log(f"[{prec}] batched: {R['ms_per_step']:.3f} ms/step, {R['volumes_per_s']:.1f} volumes/s; dominant "
@@ -287,8 +293,8 @@ def main():
         "iterations_per_level_mean": H["iterations_per_level_mean"],
         "precision": prec_text[args.precision] + "; FP64 histogram range, Wiener step, lattice update and CV sums",
         "roofline_note": "timed steps run without events; the dominant kernel's average duration comes from the "
-                         "same number of steps run again with events around each of its launches (direct "
-                         "launches: step graphs are not used while kernels are timed with events)",
+                         "same number of steps run again with one stream group and direct launches, with events "
+                         "around each of its launches",
         "schedule": H["schedule"],
         "schedule_source": {"autotune": "n4_autotune_batch on this workload (groups x tile voxels x graph steps), "
                                         "run before the warmup",
@@ -312,7 +318,7 @@ def main():
         "algorithmic_bytes_per_launch": round(H["dom_bytes"] / max(1, dom_n)),
         "avg_launch_us": round(H["dom_ms"] / max(1, dom_n) * 1e3, 3), "launches": int(dom_n),
     }
-    result["kernel_breakdown_ms_per_step"] = H["breakdown"]
+    result["kernel_ms_per_step_one_group"] = H["breakdown"]
     result["batched"] = {q: {"volumes_per_s": round(R["volumes_per_s"], 3), "ms_per_step": round(R["ms_per_step"], 4),
                              "mvoxel_iter_per_s": round(R["mvoxel_iter_per_s"], 2),
                              "logbias_err_vs_fp64": R["err"], "kernel_ms_per_step": R["breakdown"],

@@ -50,6 +50,28 @@
 #define UNROLL
 #endif
 
+// Work of wavefront 0 alone (lane l of 64), with wave-level ordering of its
+// LDS phases only (no workgroup barrier): a wave's LDS instructions execute
+// in issue order, so a phase sees the previous phase's stores.
+#ifdef N4_DEVICE_PASS
+#define WAVE0_BEGIN if (threadIdx.x < 64) {
+#define WAVE0_END }
+#define WAVE_FOR(l) { const int l = (int)threadIdx.x; (void)l;
+#define WAVE_END }
+#define WAVE_SYNC()                                         \
+    do {                                                    \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                    \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+    } while (0)
+#else
+#define WAVE0_BEGIN {
+#define WAVE0_END }
+#define WAVE_FOR(l) for (int l = 0; l < 64; ++l) {
+#define WAVE_END }
+#define WAVE_SYNC() do { } while (0)
+#endif
+
 #define HD __host__ __device__ inline
 
 namespace n4 {
@@ -631,14 +653,17 @@ HD void dft_small(D2 (&v)[R], double sg) {
     UNROLL for (int i = 0; i < R; ++i) v[i] = u[i];
 }
 
-// One Stockham pass of radix R over P points (span Ns).
+// One Stockham pass of radix R over P points (span Ns): butterfly j takes
+// a[j + r*P/R], twiddles them by exp(-+2 pi i r k / (Ns R)) (k = j mod Ns)
+// and writes b[(j - k) R + k + r Ns].
+// The same pass run by one wavefront (lane l of 64).
 template <int R>
-HD void fft_pass(const D2* a, D2* b, const D2* tw, int P, int Ns, bool inverse) {
+HD void wave_fft_pass(const D2* a, D2* b, const D2* tw, int P, int Ns, bool inverse) {
     const int nbf = P / R;
     const int tstep = P / (Ns * R);
     const double sg = inverse ? 1.0 : -1.0;
-    BLOCK_FOR(t)
-    for (int j = t; j < nbf; j += kBlock) {
+    WAVE_FOR(l)
+    for (int j = l; j < nbf; j += 64) {
         const int k = j & (Ns - 1);
         D2 v[R];
         UNROLL for (int r = 0; r < R; ++r) v[r] = a[j + r * nbf];
@@ -653,23 +678,24 @@ HD void fft_pass(const D2* a, D2* b, const D2* tw, int P, int Ns, bool inverse) 
         const int o = (j - k) * R + k;
         UNROLL for (int r = 0; r < R; ++r) b[o + r * Ns] = v[r];
     }
-    BLOCK_END
+    WAVE_END
 }
 
 // LDS FP64 Stockham FFT over P = 2^k points with radix-8 passes (radix 4
-// or 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P, in LDS.
-// Returns the buffer holding the result (the other one is free).
-HD D2* lds_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
+// or 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P, in LDS.  Run by
+// wavefront 0 alone.  Returns the buffer holding the result (the other one
+// is free).
+HD D2* wave_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
     for (int Ns = 1; Ns < P;) {
         const int rem = P / Ns;
         const int R = rem >= 8 ? 8 : rem;
         if (R == 8)
-            fft_pass<8>(a, b, tw, P, Ns, inverse);
+            wave_fft_pass<8>(a, b, tw, P, Ns, inverse);
         else if (R == 4)
-            fft_pass<4>(a, b, tw, P, Ns, inverse);
+            wave_fft_pass<4>(a, b, tw, P, Ns, inverse);
         else
-            fft_pass<2>(a, b, tw, P, Ns, inverse);
-        BLOCK_SYNC();
+            wave_fft_pass<2>(a, b, tw, P, Ns, inverse);
+        WAVE_SYNC();
         D2* tmp = a;
         a = b;
         b = tmp;
@@ -715,11 +741,13 @@ HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigne
     }
     BLOCK_END
     BLOCK_SYNC();
-    D2* Z = lds_fft(z, y, tw, P, false);
+    // the sequential chain of transforms and pointwise steps: wavefront 0
+    WAVE0_BEGIN
+    D2* Z = wave_fft(z, y, tw, P, false);
     D2* U = (Z == z) ? y : z;
     // Split Vf and Ff, Gf = conj(Ff)/(conj(Ff)Ff + noise), Uf = Vf*Re(Gf)
-    BLOCK_FOR(t)
-    for (int k = t; k < P; k += kBlock) {
+    WAVE_FOR(l)
+    for (int k = l; k < P; k += 64) {
         const D2 a = Z[k];
         const D2 m = Z[(P - k) & (P - 1)];
         const D2 Vf{0.5 * (a.x + m.x), 0.5 * (a.y - m.y)};
@@ -728,33 +756,34 @@ HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigne
         const double g = Ff.x / ((Ff.x * Ff.x + Ff.y * Ff.y) + rp.noise);
         U[k] = D2{Vf.x * g, Vf.y * g};
     }
-    BLOCK_END
-    BLOCK_SYNC();
-    D2* Ur = lds_fft(U, Z, tw, P, true);
+    WAVE_END
+    WAVE_SYNC();
+    D2* Ur = wave_fft(U, Z, tw, P, true);
     D2* Wb = (Ur == U) ? Z : U;
     const double invP = 1.0 / (double)P;
-    BLOCK_FOR(t)
-    for (int j = t; j < P; j += kBlock) {
+    WAVE_FOR(l)
+    for (int j = l; j < P; j += 64) {
         double u = Ur[j].x * invP;
         if (u < 0.0) u = 0.0;
         Wb[j] = D2{(bmin + (double)(j - off) * slope) * u, u};
     }
-    BLOCK_END
-    BLOCK_SYNC();
-    D2* Wf = lds_fft(Wb, Ur, tw, P, false);
+    WAVE_END
+    WAVE_SYNC();
+    D2* Wf = wave_fft(Wb, Ur, tw, P, false);
     D2* Wo = (Wf == Wb) ? Ur : Wb;
-    BLOCK_FOR(t)
-    for (int k = t; k < P; k += kBlock) Wf[k] = cmul(Wf[k], ff[k]);
-    BLOCK_END
-    BLOCK_SYNC();
-    D2* R = lds_fft(Wf, Wo, tw, P, true);
-    BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock) {
+    WAVE_FOR(l)
+    for (int k = l; k < P; k += 64) Wf[k] = cmul(Wf[k], ff[k]);
+    WAVE_END
+    WAVE_SYNC();
+    D2* R = wave_fft(Wf, Wo, tw, P, true);
+    WAVE_FOR(l)
+    for (int i = l; i < nb; i += 64) {
         const double nu = R[off + i].x * invP;
         const double de = R[off + i].y * invP;
         V.etab[i] = (de != 0.0) ? nu / de : 0.0;
     }
-    BLOCK_END
+    WAVE_END
+    WAVE0_END
 }
 
 // One block per volume: the step's per-volume work and the state for the
@@ -869,16 +898,22 @@ HD void body_update(const VolDesc* vols, RunParams rp, int chunk, int vol) {
 // level), FIELD (g = the logU buffer itself; stage API).
 struct FitLds {
     double* E;
-    float* zw;
-    int* zb;
+    float* zwc;  // [Cz][zmax]  factor of control point cz at z = lo(cz) + i
     int* zs;
     int* ys;
     float* g;
     float* T1;
     float* T2;
-    float* yw;
-    int* yb;
+    float* ywc;  // [Cy][ymax]  factor of control point cy at row y = lo(cy) + i
 };
+
+// Longest support (in grid points) of one cubic B-spline basis function over
+// n points with C control points: 4 spans of at most ceil(n / (C - 3)) + 1.
+HD int support_len(int n, int C) {
+    const int nint = C - 3 > 0 ? C - 3 : 1;
+    const int m = 4 * ((n + nint - 1) / nint + 1);
+    return m < n ? m : n;
+}
 
 // Resolves kFitStep from the volume's current step; false: nothing to fit.
 HD bool fit_step_mode(const VolState* st, int* mode, int* level, int* it) {
@@ -915,19 +950,17 @@ HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const double*
     return (float)((double)xv - sh);
 }
 
-HD size_t carve_fit(unsigned char* base, int nbins, int Z, int R, int Cy, int Cz, FitLds* o) {
+HD size_t carve_fit(unsigned char* base, int nbins, int Y, int Z, int R, int Cy, int Cz, FitLds* o) {
     Carve c{base, 0};
     FitLds L;
     L.E = c.take<double>(nbins);
-    L.zw = c.take<float>((size_t)4 * Z);
-    L.zb = c.take<int>(Z);
+    L.zwc = c.take<float>((size_t)Cz * support_len(Z, Cz));
     L.zs = c.take<int>((size_t)2 * Cz);
     L.ys = c.take<int>((size_t)2 * Cy);
     L.g = c.take<float>((size_t)R * Z);
     L.T1 = c.take<float>((size_t)R * Cz);
     L.T2 = c.take<float>((size_t)Cy * Cz);
-    L.yw = c.take<float>((size_t)4 * R);
-    L.yb = c.take<int>(R);
+    L.ywc = c.take<float>((size_t)Cy * support_len(Y, Cy));
     if (o) *o = L;
     return align16(c.off);
 }
@@ -944,7 +977,8 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     const int R = rows_per_chunk(Y, Z);
     const int nb = rp.nbins;
     FitLds S;
-    carve_fit(smem, nb, Z, R, Cy, Cz, &S);
+    carve_fit(smem, nb, Y, Z, R, Cy, Cz, &S);
+    const int zmax = support_len(Z, Cz), ymax = support_len(Y, Cy);
     const bool num = (mode == kFitNum);
     const bool den = (mode == kFitDen || mode == kFitDenScatter);
     const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
@@ -954,13 +988,32 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     const AxisEntry* axz = LD.ax[2];
     const AxisEntry* axy = LD.ax[1];
     const int x = sl.x;
+    // per control point, its factors over its support as one contiguous run
+    // (the contraction loops then read two independent LDS streams)
     BLOCK_FOR(t)
     if (num)
         for (int i = t; i < nb; i += kBlock) S.E[i] = V.etab[i];
-    for (int z = t; z < Z; z += kBlock) {
-        const AxisEntry& e = axz[z];
-        for (int a = 0; a < 4; ++a) S.zw[4 * z + a] = den ? e.q[a] : e.p[a];
-        S.zb[z] = e.b;
+    for (int e = t; e < Cz * zmax; e += kBlock) {
+        const int c = e / zmax, i = e - c * zmax;
+        const int z = LD.sup[2][2 * c] + i;
+        float w = 0.0f;
+        if (z < LD.sup[2][2 * c + 1]) {
+            const AxisEntry& a = axz[z];
+            const int k = c - a.b;
+            w = den ? a.q[k] : a.p[k];
+        }
+        S.zwc[e] = w;
+    }
+    for (int e = t; e < Cy * ymax; e += kBlock) {
+        const int c = e / ymax, i = e - c * ymax;
+        const int y = LD.sup[1][2 * c] + i;
+        float w = 0.0f;
+        if (y < LD.sup[1][2 * c + 1]) {
+            const AxisEntry& a = axy[y];
+            const int k = c - a.b;
+            w = den ? a.q[k] : a.p[k];
+        }
+        S.ywc[e] = w;
     }
     for (int i = t; i < 2 * Cz; i += kBlock) S.zs[i] = LD.sup[2][i];
     for (int i = t; i < 2 * Cy; i += kBlock) S.ys[i] = LD.sup[1][i];
@@ -972,11 +1025,6 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         const int ny = (yend - y0 < R) ? yend - y0 : R;
         const size_t base = ((size_t)x * Y + y0) * Z;
         BLOCK_FOR(t)
-        for (int r = t; r < ny; r += kBlock) {
-            const AxisEntry& e = axy[y0 + r];
-            for (int a = 0; a < 4; ++a) S.yw[4 * r + a] = den ? e.q[a] : e.p[a];
-            S.yb[r] = e.b;
-        }
         // aligned voxel quads covering the chunk (16-byte logU loads)
         const size_t vend = base + (size_t)ny * Z, q0 = base & ~(size_t)3;
         const int nq = (int)((vend - q0 + 3) >> 2);
@@ -1005,10 +1053,11 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
-            const int lo = S.zs[2 * c], hi = S.zs[2 * c + 1];
-            const float* gr = S.g + (size_t)r * Z;
+            const int lo = S.zs[2 * c], n = S.zs[2 * c + 1] - lo;
+            const float* gr = S.g + (size_t)r * Z + lo;
+            const float* wc = S.zwc + (size_t)c * zmax;
             float acc = 0.0f;
-            for (int z = lo; z < hi; ++z) acc += gr[z] * S.zw[4 * z + (c - S.zb[z])];
+            for (int i = 0; i < n; ++i) acc += gr[i] * wc[i];
             S.T1[pr] = acc;
         }
         BLOCK_END
@@ -1016,14 +1065,13 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         BLOCK_FOR(t)
         for (int pr = t; pr < Cy * Cz; pr += kBlock) {
             const int cy = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - cy * Cz;
-            int lo = S.ys[2 * cy], hi = S.ys[2 * cy + 1];
+            const int ylo = S.ys[2 * cy];
+            int lo = ylo, hi = S.ys[2 * cy + 1];
             lo = lo > y0 ? lo : y0;
             hi = hi < y0 + ny ? hi : y0 + ny;
+            const float* wc = S.ywc + (size_t)cy * ymax - ylo;
             float acc = S.T2[pr];
-            for (int yy = lo; yy < hi; ++yy) {
-                const int r = yy - y0;
-                acc += S.yw[4 * r + (cy - S.yb[r])] * S.T1[r * Cz + c];
-            }
+            for (int yy = lo; yy < hi; ++yy) acc += wc[yy] * S.T1[(yy - y0) * Cz + c];
             S.T2[pr] = acc;
         }
         BLOCK_END
@@ -1436,7 +1484,7 @@ size_t lds_fit(int nbins, const VolDesc& v, int level, int prec) {
                               nullptr);
 #endif
     (void)prec;
-    return carve_fit(nullptr, nbins, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
+    return carve_fit(nullptr, nbins, v.W[1], v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
 }
 
 size_t lds_eval(const VolDesc& v, int level, int prec) {
