@@ -233,11 +233,12 @@ def main():
         lib.synchronize()
         dt = time.perf_counter() - t0
         # kernel durations, measured apart from the timed region with one
-        # stream group and direct launches (no concurrent groups sharing the
-        # CUs, no graphs): one step with events around every launch picks
+        # stream group, direct launches and the default tiles (no concurrent
+        # groups sharing the CUs, no graphs; the same launches whatever the
+        # schedule of the timed region): one step with events around every launch picks
         # the dominant kernel, then the same number of steps as timed with
         # events around each of its launches give its average duration
-        lib.set_tuning(**dict(schedule, groups=1, graph_steps="off"))
+        lib.set_tuning(groups=1, graph_steps="off")  # library defaults otherwise
         lib.profile_enable(n4amd.KERNELS)
         batch_step(p)
         lib.synchronize()
@@ -293,8 +294,9 @@ def main():
         "iterations_per_level_mean": H["iterations_per_level_mean"],
         "precision": prec_text[args.precision] + "; FP64 histogram range, Wiener step, lattice update and CV sums",
         "roofline_note": "timed steps run without events; the dominant kernel's average duration comes from the "
-                         "same number of steps run again with one stream group and direct launches, with events "
-                         "around each of its launches",
+                         "same number of steps run again with one stream group, direct launches and the default "
+                         "tiles (independent of the timed region's schedule), with events around each of its "
+                         "launches",
         "schedule": H["schedule"],
         "schedule_source": {"autotune": "n4_autotune_batch on this workload (groups x tile voxels x graph steps), "
                                         "run before the warmup",
@@ -302,22 +304,35 @@ def main():
                                             args.schedule],
     }
     dominant, dom_n = H["dominant"], H["dom_n"]
-    traffic, traffic_src = None, None
-    tpath = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        t = tj.get("per_launch", {}).get(args.precision, {}).get(dominant)
-        if t and t.get("fetch_bytes") is not None:
-            traffic = round(t["fetch_bytes"] + (t.get("write_bytes") or 0.0))
-            traffic_src = tj.get("source")
+    # HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same
+    # kernel-timing dispatches, scaled by a copy of known size) and the
+    # profiler's own average duration of those dispatches, from the committed
+    # profile of this command
+    traffic, traffic_src, prof_us = None, None, None
+    ppath = os.path.join(REPO, "profiles", f"{args.workload}_{args.precision}_profile.json")
+    if os.path.exists(ppath):
+        with open(ppath) as f:
+            pj = json.load(f)
+        tp = pj.get("kernel_timing_pass", {})
+        if tp.get("kernel") == dominant:
+            if tp.get("fetch_bytes") is not None:
+                traffic = round(tp["fetch_bytes"] + (tp.get("write_bytes") or 0.0))
+            prof_us = tp.get("rocprof_avg_us")
+            traffic_src = os.path.relpath(ppath, REPO)
+    avg_us = H["dom_ms"] / max(1, dom_n) * 1e3
+    bytes_per_launch = H["dom_bytes"] / max(1, dom_n)
     result["roofline"] = {
         "bound": "hbm", "kernel": dominant, "achieved": round(H["achieved"], 2), "peak": PEAK_HBM_GBS,
         "unit": "GB/s", "frac": round(H["achieved"] / PEAK_HBM_GBS, 5), "traffic": traffic,
         "traffic_source": traffic_src,
-        "algorithmic_bytes_per_launch": round(H["dom_bytes"] / max(1, dom_n)),
-        "avg_launch_us": round(H["dom_ms"] / max(1, dom_n) * 1e3, 3), "launches": int(dom_n),
+        "algorithmic_bytes_per_launch": round(bytes_per_launch),
+        "avg_launch_us": round(avg_us, 3), "launches": int(dom_n),
     }
+    if prof_us:
+        # HIP event pairs around a launch also hold the event markers and
+        # the dispatch; the profiler times the dispatch's execution alone
+        result["roofline"]["rocprof_avg_launch_us"] = round(prof_us, 3)
+        result["roofline"]["achieved_at_rocprof_duration"] = round(bytes_per_launch / (prof_us * 1e-6) / 1e9, 2)
     result["kernel_ms_per_step_one_group"] = H["breakdown"]
     result["batched"] = {q: {"volumes_per_s": round(R["volumes_per_s"], 3), "ms_per_step": round(R["ms_per_step"], 4),
                              "mvoxel_iter_per_s": round(R["mvoxel_iter_per_s"], 2),

@@ -645,9 +645,12 @@ struct ProfScope {
 #endif
     }
 #ifndef N4_EMULATE
+    // Timing events without the system-scope fence an event record
+    // performs by default (a cache flush that would stretch the measured
+    // interval and slow the kernel after it).
     void record() {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
         (void)hipEventRecord(e, (hipStream_t)stream);
         h->prof_ev.push_back(e);
     }

@@ -226,7 +226,11 @@ typedef enum n4_kernel_id {
     N4_K_COUNT = 10
 } n4_kernel_id;
 /* Time every launch of the kernels in kernel_mask (bit = n4_kernel_id) with
- * HIP events on the launch stream; clears the accumulators. 0 disables. */
+ * a pair of HIP events recorded on the launch stream around it (step graphs
+ * are not used meanwhile); clears the accumulators. 0 disables.  The
+ * interval includes the event markers and the dispatch of the launch, so
+ * for kernels of tens of microseconds it exceeds the execution time a
+ * profiler reports. */
 n4_status n4_profile_enable(n4_handle *h, uint32_t kernel_mask);
 /* Accumulated device time and launch count of one kernel since enable. */
 n4_status n4_profile_read(n4_handle *h, uint32_t kernel, double *total_ms, uint64_t *launches);
