@@ -224,6 +224,14 @@ HD double cv_from_sums(double s1, double s2, double n) {
     return sqrt(var) / (1.0 + mu);
 }
 
+// Rows of the evaluation kernel's chunk: its staging is only Cz floats per
+// row, so a chunk covers a whole default tile.
+HD int rows_per_eval_chunk(int Y, int Z) {
+    int r = 4 * kChunkVox / (Z > 0 ? Z : 1);
+    if (r < 1) r = 1;
+    return r < Y ? r : Y;
+}
+
 HD int rows_per_chunk(int Y, int Z) {
     int r = kChunkVox / (Z > 0 ? Z : 1);
     if (r < 1) r = 1;
@@ -897,7 +905,7 @@ HD void body_update(const VolDesc* vols, RunParams rp, int chunk, int vol) {
 // Modes: NUM (g = residual, P = w^3/S), DEN (g = mask, Q = w^2; once per
 // level), FIELD (g = the logU buffer itself; stage API).
 struct FitLds {
-    double* E;
+    float* E;
     float* zwc;  // [Cz][zmax]  factor of control point cz at z = lo(cz) + i
     int* zs;
     int* ys;
@@ -933,27 +941,41 @@ HD bool fit_step_mode(const VolState* st, int* mode, int* level, int* it) {
     return false;
 }
 
+// The remap of a step in FP32: the E table (rounded from the FP64 Wiener
+// output), the histogram origin and the reciprocal bin width.  FP32 here
+// moves the residual by about one FP32 ulp of logU (~3e-7 at |logU| ~ 5),
+// the rounding logU itself carries; the histogram bin index, which must be
+// exact, stays FP64 in the histogram kernel.
+struct Remap {
+    const float* E;
+    float bmin, islope;
+    int nb;
+};
+
+HD Remap make_remap(const float* E, double bmin, double slope, int nb) {
+    return Remap{E, (float)bmin, (float)(1.0 / slope), nb};
+}
+
 // Contracted quantity g of one working voxel: the residual logU - E[u|v]
 // (N4.py:103-115 remap with the cudaN4.cu:242 clamp, then N4.py:313) for
 // NUM, the mask for DEN, logU itself for FIELD; 0 outside the mask.
-HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const double* E, double bmin, double slope,
-                   int nb) {
+HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const Remap& R) {
     if (!m) return 0.0f;
     if (mode == kFitDen) return 1.0f;
     if (mode == kFitField) return xv;
     if (degenerate) return 0.0f;
-    double c = ((double)xv - bmin) / slope;
-    c = fmin(fmax(c, 0.0), (double)nb);
-    const double fl = floor(c);
-    const int idx = (int)fl;
-    const double sh = (idx < nb - 1) ? E[idx] + (E[idx + 1] - E[idx]) * (c - fl) : E[nb - 1];
-    return (float)((double)xv - sh);
+    float c = (xv - R.bmin) * R.islope;
+    c = fminf(fmaxf(c, 0.0f), (float)R.nb);
+    const int idx = (int)c;  // c >= 0: truncation is floor
+    const float fr = c - (float)idx;
+    const float sh = (idx < R.nb - 1) ? R.E[idx] + (R.E[idx + 1] - R.E[idx]) * fr : R.E[R.nb - 1];
+    return xv - sh;
 }
 
 HD size_t carve_fit(unsigned char* base, int nbins, int Y, int Z, int R, int Cy, int Cz, FitLds* o) {
     Carve c{base, 0};
     FitLds L;
-    L.E = c.take<double>(nbins);
+    L.E = c.take<float>(nbins);
     L.zwc = c.take<float>((size_t)Cz * support_len(Z, Cz));
     L.zs = c.take<int>((size_t)2 * Cz);
     L.ys = c.take<int>((size_t)2 * Cy);
@@ -985,6 +1007,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     const int gmode = den ? (int)kFitDen : mode;
     const bool degenerate = num && st->degenerate;
     const double bmin = st->bin_min, slope = st->slope;
+    const Remap rm = make_remap(S.E, bmin, slope, nb);
     const AxisEntry* axz = LD.ax[2];
     const AxisEntry* axy = LD.ax[1];
     const int x = sl.x;
@@ -992,7 +1015,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     // (the contraction loops then read two independent LDS streams)
     BLOCK_FOR(t)
     if (num)
-        for (int i = t; i < nb; i += kBlock) S.E[i] = V.etab[i];
+        for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
     for (int e = t; e < Cz * zmax; e += kBlock) {
         const int c = e / zmax, i = e - c * zmax;
         const int z = LD.sup[2][2 * c] + i;
@@ -1044,7 +1067,7 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
                     const size_t w = wq + j;
                     if (w < base || w >= vend) continue;
                     const uint8_t m = (uint8_t)((ms[u] >> (8 * j)) & 0xffu);
-                    S.g[w - base] = fit_input(gmode, m, xs[u].v[j], degenerate, S.E, bmin, slope, nb);
+                    S.g[w - base] = fit_input(gmode, m, xs[u].v[j], degenerate, rm);
                 }
             }
         }
@@ -1186,7 +1209,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
     const int CyCz = Cy * Cz;
-    const int R = rows_per_chunk(Y, Z);
+    const int R = rows_per_eval_chunk(Y, Z);
     EvalLds S;
     carve_eval(smem, Z, R, Cy, Cz, &S);
     const int x = sl.x;
@@ -1225,7 +1248,8 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
         BLOCK_SYNC();
         BLOCK_FOR(t)
         double s1 = S.red[t], s2 = S.red[kBlock + t];
-        double mn = S.red[2 * kBlock + t], mx = S.red[3 * kBlock + t];
+        // min/max of FP32 values are exact in FP32
+        float mn = (float)S.red[2 * kBlock + t], mx = (float)S.red[3 * kBlock + t];
         // consecutive lanes on consecutive voxels (conflict-free LDS
         // reads of the z tables), eight loads in flight per thread
         for (int v0 = 0; v0 < ny * Z; v0 += 8 * kBlock) {
@@ -1249,8 +1273,8 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
                 const double uu = (double)expm1f(-d);
                 s1 += uu;
                 s2 += uu * uu;
-                mn = fmin(mn, (double)lg);
-                mx = fmax(mx, (double)lg);
+                mn = fminf(mn, lg);
+                mx = fmaxf(mx, lg);
             }
         }
         S.red[t] = s1;
@@ -1495,7 +1519,7 @@ size_t lds_eval(const VolDesc& v, int level, int prec) {
                                prec, nullptr);
 #endif
     (void)prec;
-    return carve_eval(nullptr, v.W[2], rows_per_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
+    return carve_eval(nullptr, v.W[2], rows_per_eval_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
 }
 
 size_t lds_recon(const VolDesc& v) {

@@ -132,7 +132,7 @@ __device__ inline void fill_axis(const Tab& T, bool cp_rows, int pos, int ncp_pa
 
 // ------------------------------------------------------------------- fit --
 struct FitMfmaLds {
-    double* E;
+    float* E;
     Tab Pz;   // [NT*16][Zp + 8]  Pz^T: row = control point, column = z
     Tab Py;   // [MT*16][Yp + 4]  Py^T: row = control point, column = y
     float* red;
@@ -148,7 +148,7 @@ constexpr int kGStride = 72;  // halves per staged row (144 B: rows stay 16-B al
 HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
     Carve c{base, 0};
     FitMfmaLds L;
-    L.E = c.take<double>(nbins);
+    L.E = c.take<float>(nbins);
     const int zs = round_up(Z, 32) + 8, ys = round_up(Y, 16) + 4;
     L.Pz.stride = zs;
     L.Pz.hi = c.take<unsigned short>((size_t)NT * 16 * zs);
@@ -188,11 +188,45 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     const int Yt = sl.ny;  // the tile's rows sl.y0 .. sl.y0 + Yt (Py rows are tile-local)
     const int Zp = round_up(Z, 32), Yp = round_up(Yt, 16);
 
+    const Remap rm = make_remap(S.E, bmin, slope, nb);
+    // the first block of voxels (row batch 0, z chunk 0) is loaded before
+    // the table set-up, so the two global round trips overlap
+    uint8_t ms[16];
+    float xs[16];
+    auto load_rows = [&](int y0, int zc) {
+        const int z = zc + lane;
+        UNROLL for (int r = 0; r < 16; ++r) {
+            const bool ok = z < Z && y0 + r < Yt;
+            const size_t v = ((size_t)x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
+            ms[r] = ok ? V.wmask[v] : 0;
+            xs[r] = (ok && !den) ? V.logU[v] : 0.0f;
+        }
+    };
+    load_rows(16 * wv, 0);
     if (num)
-        for (int i = t; i < nb; i += kBlock) S.E[i] = V.etab[i];
-    for (int z = t; z < Zp; z += kBlock) fill_axis(S.Pz, true, z, NT * 16, Cz, z < Z ? LD.ax[2] + z : nullptr, kf);
-    for (int y = t; y < Yp; y += kBlock)
-        fill_axis(S.Py, true, y, MT * 16, Cy, y < Yt ? LD.ax[1] + sl.y0 + y : nullptr, kf);
+        for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
+    // factor tables, one entry per thread: consecutive threads fill
+    // consecutive positions of a control point's row
+    for (int e = t; e < NT * 16 * Zp; e += kBlock) {
+        const int c = e / Zp, z = e - c * Zp;
+        float v = 0.0f;
+        if (z < Z && c < Cz) {
+            const AxisEntry& a = LD.ax[2][z];
+            const int k = c - a.b;
+            if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
+        }
+        S.Pz.put(c, z, v);
+    }
+    for (int e = t; e < MT * 16 * Yp; e += kBlock) {
+        const int c = e / Yp, y = e - c * Yp;
+        float v = 0.0f;
+        if (y < Yt && c < Cy) {
+            const AxisEntry& a = LD.ax[1][sl.y0 + y];
+            const int k = c - a.b;
+            if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
+        }
+        S.Py.put(c, y, v);
+    }
     __syncthreads();
 
     f32x4 acc[MT][NT];
@@ -206,17 +240,9 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
         f32x4 t1[NT];
         UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
         for (int zc = 0; zc < Z; zc += 64) {
-            const int z = zc + lane;
-            uint8_t ms[16];
-            float xs[16];
+            if (rb > 0 || zc > 0) load_rows(y0, zc);  // (the first block is already in registers)
             UNROLL for (int r = 0; r < 16; ++r) {
-                const bool ok = z < Z && y0 + r < Yt;
-                const size_t v = ((size_t)x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
-                ms[r] = ok ? V.wmask[v] : 0;
-                xs[r] = (ok && !den) ? V.logU[v] : 0.0f;
-            }
-            UNROLL for (int r = 0; r < 16; ++r) {
-                const float gv = fit_input(gmode, ms[r], xs[r], degenerate, S.E, bmin, slope, nb);
+                const float gv = fit_input(gmode, ms[r], xs[r], degenerate, rm);
                 unsigned short h, l = 0;
                 if constexpr (PREC == 2)
                     bf16_split(gv, h, l);
@@ -389,7 +415,8 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
     const int Yt = sl.ny;  // the tile's rows (Wy rows are tile-local)
     // the 4 increment planes slab x touches (FP32, written by the update kernel)
     eval_mfma_setup<MT, NT>(S, ex, V.lresf + (size_t)ex.b * Cy * Cz, LD.ax[1], sl.y0, Yt, LD.ax[2], Z, Cy, Cz);
-    double s1 = 0.0, s2 = 0.0, mn = INFINITY, mx = -INFINITY;
+    double s1 = 0.0, s2 = 0.0;
+    float mn = INFINITY, mx = -INFINITY;  // exact in FP32
     for (int y0 = 16 * wv; y0 < Yt; y0 += 64) {
         i16x4 bh[NT], bl[NT];
         eval_mfma_rows<PREC, MT, NT>(S, y0, r16, g4, bh, bl);
@@ -412,8 +439,8 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
                 const double uu = (double)expm1f(-d[i]);
                 s1 += uu;
                 s2 += uu * uu;
-                mn = fmin(mn, (double)lg);
-                mx = fmax(mx, (double)lg);
+                mn = fminf(mn, lg);
+                mx = fmaxf(mx, lg);
             }
         }
     }
