@@ -195,16 +195,30 @@ HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
 #endif
 }
 
-// Reduces n partial quadruples part[i] = (s0, s1, min, max).
+// Reduces n partial quadruples part[i] = (s0, s1, min, max).  Thread t
+// folds partials t, t + kBlock, ... in that order; their loads are issued
+// eight at a time (two 16-byte loads per partial) before they are folded.
+struct alignas(16) D2x {
+    double a, b;
+};
 HD void reduce_partials(const double* part, int nslab, double* red) {
     BLOCK_FOR(t)
     double r0 = 0.0, r1 = 0.0, r2 = INFINITY, r3 = -INFINITY;
-    for (int xx = t; xx < nslab; xx += kBlock) {
-        const double* sp = part + (size_t)xx * 4;
-        r0 += sp[0];
-        r1 += sp[1];
-        r2 = fmin(r2, sp[2]);
-        r3 = fmax(r3, sp[3]);
+    for (int x0 = t; x0 < nslab; x0 += 8 * kBlock) {
+        D2x lo[8], hi[8];
+        UNROLL for (int u = 0; u < 8; ++u) {
+            const int xx = x0 + u * kBlock;
+            const D2x* sp = reinterpret_cast<const D2x*>(part + (size_t)(xx < nslab ? xx : 0) * 4);
+            lo[u] = sp[0];
+            hi[u] = sp[1];
+        }
+        UNROLL for (int u = 0; u < 8; ++u) {
+            if (x0 + u * kBlock >= nslab) break;
+            r0 += lo[u].a;
+            r1 += lo[u].b;
+            r2 = fmin(r2, hi[u].a);
+            r3 = fmax(r3, hi[u].b);
+        }
     }
     red[t] = r0;
     red[kBlock + t] = r1;
