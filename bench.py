@@ -55,19 +55,31 @@ def make_workload(name):
                            "bias lattice 4^3-6^3 N(0,0.3), Rician sigma U[2,8], shrink 4, "
                            "4 levels x <=50 iterations, threshold 0.001",
                "volumes": 64, "shrink": 4, "levels": 4, "max_iterations": 50}
-        return vols, cfg, 4
+        return vols, cfg, [4] * 64
+    if name == "batch256":
+        specs = P.batch_specs(256, 96, 320, seed=4)
+        shrinks = P.batch_shrinks(256, seed=5)
+        from concurrent.futures import ProcessPoolExecutor
+        workers = min(8, os.cpu_count() or 1)
+        with ProcessPoolExecutor(workers) as ex:
+            vols = list(ex.map(_gen, specs))
+        cfg = {"workload": "batch of 256 head-class phantoms, axes U{96..320} (seed 4), mask 30-60%, "
+                           "bias lattice 4^3-6^3 N(0,0.3), Rician sigma U[2,8], shrink U{2,3,4} per volume "
+                           "(seed 5), 4 levels x <=50 iterations, threshold 0.001",
+               "volumes": 256, "shrink": "2-4 per volume", "levels": 4, "max_iterations": 50}
+        return vols, cfg, shrinks
     if name == "c1":
         im, mask, _ = P.c1_phantom(1)
         cfg = {"workload": "256x256x176 head phantom (mask ~45%), bias lattice 5^3 N(0,0.3), shrink 4 "
                            "(64x64x44 working grid), 4 levels x <=50 iterations, threshold 0.001",
                "volumes": 1, "shrink": 4, "levels": 4, "max_iterations": 50}
-        return [(im, mask)], cfg, 4
+        return [(im, mask)], cfg, [4]
     if name == "c2":
         im, mask, _ = P.c2_phantom(2)
         cfg = {"workload": "512x512x320 7T-like head phantom, bias lattice 6^3 N(0,0.4), shrink 2 "
                            "(256x256x160 working grid), 4 levels x <=50 iterations, threshold 0.001",
                "volumes": 1, "shrink": 2, "levels": 4, "max_iterations": 50}
-        return [(im, mask)], cfg, 2
+        return [(im, mask)], cfg, [2]
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -97,7 +109,7 @@ def algorithmic_bytes(kernel, vols_meta, reports, levels):
     return total
 
 
-def oracle_sample(vols, cfg, shrink, cpu_seconds, nthreads):
+def oracle_sample(vols, cfg, shrinks, cpu_seconds, nthreads):
     """FP64 oracle on the first volumes of the workload until cpu_seconds of
     wall time: the CPU baseline and the error reference."""
     import oracle as O
@@ -105,7 +117,7 @@ def oracle_sample(vols, cfg, shrink, cpu_seconds, nthreads):
     for i, (im, mask) in enumerate(vols):
         t1 = time.perf_counter()
         ro = O.n4(im, mask, levels=cfg["levels"], max_iter=[cfg["max_iterations"]] * cfg["levels"],
-                  conv_threshold=0.001, shrink=shrink, nthreads=nthreads)
+                  conv_threshold=0.001, shrink=shrinks[i], nthreads=nthreads)
         dt = time.perf_counter() - t1
         t_cpu += dt
         out.append({"i": i, "log_bias": ro["log_bias"], "iterations": ro["iterations"]})
@@ -120,7 +132,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch64", choices=["batch64", "c1", "c2"])
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "c1", "c2", "batch256"])
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "split_bf16"],
                     help="precision of the headline batched figure")
     ap.add_argument("--precisions", default="fp32,bf16,split_bf16",
@@ -146,13 +158,13 @@ def main():
     if not args.copy_calibration_mib:
         # generate in worker processes before this process touches the GPU
         t0 = time.time()
-        vols, cfg, shrink = make_workload(args.workload)
+        vols, cfg, shrinks = make_workload(args.workload)
         log(f"generated {len(vols)} volume(s) in {time.time() - t0:.1f}s")
         # one core for the batch (N4.py's execution model); all host cores (at
         # most 16) for the single large volume, which one core cannot finish
         cpu_threads = 1 if len(vols) > 1 else min(16, os.cpu_count() or 1)
         sample, t_cpu = (([], 0.0) if args.no_cpu_baseline
-                         else oracle_sample(vols, cfg, shrink, args.cpu_seconds, cpu_threads))
+                         else oracle_sample(vols, cfg, shrinks, args.cpu_seconds, cpu_threads))
     lib = n4amd.load()
     log(f"library {lib.version()} ({lib.path})")
     if args.copy_calibration_mib:
@@ -164,7 +176,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return
 
-    base_kw = dict(levels=cfg["levels"], max_iter=cfg["max_iterations"], threshold=0.001, shrink=shrink)
+    base_kw = dict(levels=cfg["levels"], max_iter=cfg["max_iterations"], threshold=0.001, shrink=shrinks[0])
     nc = n4amd.final_ncpt(n4amd.default_params(**base_kw))
     L = nc[0] * nc[1] * nc[2]
     # device-resident inputs and outputs
@@ -181,9 +193,9 @@ def main():
         v.d_im, v.d_mask, v.mask_kind = d_im.ptr, d_mask.ptr, n4amd.MASK_F32
         for d in range(3):
             v.dims[d] = im.shape[d]
-        v.shrink_factor = 0
+        v.shrink_factor = shrinks[i]
         v.d_im_normalized, v.d_lattice, v.d_bias_field, v.d_log_bias_field = d_out.ptr, d_lat.ptr, None, d_lb.ptr
-        W = [(s + shrink - 1) // shrink for s in im.shape]
+        W = [(s + shrinks[i] - 1) // shrinks[i] for s in im.shape]
         meta.append({"n_full": int(im.size), "n_work": int(np.prod(W)), "dims": im.shape})
     nvol = len(vols)
     reps = (n4amd.Report * nvol)()
@@ -373,6 +385,7 @@ def main():
         def plain_pass():
             for i in range(nvol):
                 v = arr[i]
+                p32.shrink_factor = shrinks[i]
                 lib._check(lib.L.n4_correct(lib.h, v.d_im, v.d_mask, v.mask_kind, v.dims[0], v.dims[1], v.dims[2],
                                             C.byref(p32), v.d_im_normalized, v.d_lattice, None,
                                             v.d_log_bias_field, C.byref(rep1), None))

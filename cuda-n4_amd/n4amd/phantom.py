@@ -147,3 +147,10 @@ def mask_fraction(mask) -> float:
 def analytic_ellipsoid_count(semi_axes) -> float:
     a, b, c = semi_axes
     return 4.0 / 3.0 * math.pi * a * b * c
+
+
+def batch_shrinks(n: int, seed: int, choices=(2, 3, 4)):
+    """Per-volume shrink factors for the mixed-shrink batch, drawn uniformly
+    from `choices` with their own seeded generator."""
+    rng = np.random.default_rng(seed)
+    return [int(v) for v in rng.choice(np.asarray(choices), size=n)]

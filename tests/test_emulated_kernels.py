@@ -135,3 +135,16 @@ def test_autotune_picks_a_candidate(emu):
             emu.autotune_batch([], levels=2)
     finally:
         emu.set_tuning(reset=True)
+
+
+def test_mixed_shrink_batch(emu):
+    """One batch whose volumes carry their own shrink factors gives each
+    volume the oracle's result at its own shrink."""
+    import oracle as O
+    dims, shr = [(30, 26, 22), (33, 29, 27), (36, 30, 28)], [2, 3, 4]
+    vols = [PC.small_phantom(d, seed=90 + k)[:2] for k, d in enumerate(dims)]
+    out = emu.correct_batch(vols, levels=2, max_iter=[4, 4], fixed_iterations=True, shrink=shr)
+    for (im, m), s, r in zip(vols, shr, out):
+        ro = O.n4(im, m, levels=2, max_iter=[4, 4], conv_threshold=0.0, shrink=s)
+        mm = (np.asarray(m) != 0) & (im > 10)
+        assert np.max(np.abs(r["log_bias"].astype(np.float64) - ro["log_bias"])[mm]) <= 1e-4

@@ -106,3 +106,24 @@ def test_precision_limits(gpu):
         gpu.correct(im, m, levels=6, ncpt=(4, 4, 5), precision="bf16")  # 5 -> 67 along z
     with pytest.raises(n4amd.N4Error):
         gpu.correct(im, m, precision=7)
+
+
+def test_split_mixed_shrink_batch_vs_oracle(gpu):
+    """The 256-volume configuration's form at small size: one batch whose
+    volumes carry their own shrink factors (2, 3, 4), split-BF16, each volume
+    against the FP64 oracle at its own shrink."""
+    dims = [(64, 56, 48), (72, 60, 54), (80, 64, 60), (50, 62, 44)]
+    shr = [2, 3, 4, 2]
+    vols = []
+    for k, d in enumerate(dims):
+        im, m, _ = PC.small_phantom(d, seed=80 + k)
+        vols.append((im, m))
+    print("split_bf16: mixed-shrink batch of 4", flush=True)
+    out = gpu.correct_batch(vols, levels=3, max_iter=[12] * 3, threshold=0.001, shrink=shr, precision="split_bf16")
+    for (im, m), s, r in zip(vols, shr, out):
+        ro = O.n4(im, m, levels=3, max_iter=[12] * 3, conv_threshold=0.001, shrink=s)
+        assert r["iterations"] == ro["iterations"], (s, r["iterations"], ro["iterations"])
+        mm = (np.asarray(m) != 0) & (im > 10)
+        err = np.abs(r["log_bias"].astype(np.float64) - ro["log_bias"])[mm]
+        print(f"shrink {s}: max err {err.max():.3g}")
+        assert err.max() <= TOL["split_bf16"]
