@@ -96,15 +96,6 @@ HD void global_add_u64(uint64_t* p, unsigned long long v) {
 #endif
 }
 
-// Store of a value no kernel reads again (final outputs).
-HD void store_stream(float* p, float v) {
-#ifdef N4_DEVICE_PASS
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
 HD void sincospi_d(double x, double* s, double* c) {
 #ifdef N4_DEVICE_PASS
     sincospi(x, s, c);
@@ -320,74 +311,94 @@ HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, i
         }
 }
 
+// Writes one working voxel (block of nb full voxels, nm of them masked with
+// log sum acc) and folds it into the thread's partials.
+HD void prep_commit(const VolDesc& V, int x, int J, int K, int nm, int nb, float acc, double& nfull, double& nwork,
+                    double& mn, double& mx) {
+    const size_t w = ((size_t)x * V.W[1] + J) * V.W[2] + K;
+    const bool fg = nm > 0 && 2 * nm >= nb;
+    const float lg = nm > 0 ? acc / (float)nm : 0.0f;
+    V.imlog[w] = lg;
+    V.logU[w] = lg;
+    V.wmask[w] = fg ? 1 : 0;
+    nfull += nm;
+    if (fg) {
+        nwork += 1.0;
+        mn = fmin(mn, (double)lg);
+        mx = fmax(mx, (double)lg);
+    }
+}
+
 template <int S, int MK>
 HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsigned char* smem) {
     Carve cv{smem, 0};
     double* red = cv.take<double>(4 * kBlock);
-    const int W1 = V.W[1], W2 = V.W[2];
+    const int W2 = V.W[2];
     const int x = sl.x;
-    // Shrink 1..3: UN working voxels per thread have all their s^3 image and
-    // mask loads in flight before any is folded (same folding order as
-    // prep_voxel); shrink 4 and boundary blocks go through prep_voxel.
-    constexpr bool kBatched = S >= 1 && S <= 3;
-    constexpr int UN = S == 1 ? 8 : (S == 2 ? 4 : 1);
-    constexpr int SB = kBatched ? S : 1;
-    constexpr int NV = SB * SB * SB;
-    const int F0 = V.F[0], F1 = V.F[1], F2 = V.F[2];
-    const int n = sl.ny * W2;
+    // Shrink 1 and 2: UN working voxels per thread have all their s^3 image
+    // and mask loads in flight before any is folded (same folding order as
+    // prep_voxel).  Larger shrinks keep one working voxel per iteration
+    // (their blocks already hold 27+ loads; the shrink cases share one
+    // kernel, whose register count is that of its largest case).
+    constexpr bool kBatched = S >= 1 && S <= 2;
     BLOCK_FOR(t)
     double nfull = 0.0, nwork = 0.0;
     double mn = INFINITY, mx = -INFINITY;
-    for (int v0 = t; v0 < n; v0 += UN * kBlock) {
-        float xv[UN][NV];
-        bool mv[UN][NV];
-        bool in[UN];
-        UNROLL for (int u = 0; u < UN; ++u) {
-            const int v = v0 + u * kBlock;
-            in[u] = false;
-            if (kBatched && v < n) {
-                const int jr = (int)fdiv((uint32_t)v, V.mW2);
-                const int i0 = x * SB, j0 = (sl.y0 + jr) * SB, k0 = (v - jr * W2) * SB;
-                in[u] = (i0 + SB <= F0) && (j0 + SB <= F1) && (k0 + SB <= F2);
-                if (in[u]) {
-                    UNROLL for (int a = 0; a < SB; ++a) UNROLL for (int b = 0; b < SB; ++b) UNROLL for (int c = 0; c < SB; ++c) {
-                        const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
-                        xv[u][(a * SB + b) * SB + c] = V.im[fv];
-                        mv[u][(a * SB + b) * SB + c] = prep_mask<S, MK>(V, fv);
+    if constexpr (kBatched) {
+        constexpr int UN = S == 1 ? 8 : 2;
+        constexpr int SB = kBatched ? S : 1;
+        constexpr int NV = SB * SB * SB;
+        const int F0 = V.F[0], F1 = V.F[1], F2 = V.F[2];
+        const int n = sl.ny * W2;
+        for (int v0 = t; v0 < n; v0 += UN * kBlock) {
+            float xv[UN][NV];
+            bool mv[UN][NV];
+            bool in[UN];
+            UNROLL for (int u = 0; u < UN; ++u) {
+                const int v = v0 + u * kBlock;
+                in[u] = false;
+                if (v < n) {
+                    const int jr = (int)fdiv((uint32_t)v, V.mW2);
+                    const int i0 = x * SB, j0 = (sl.y0 + jr) * SB, k0 = (v - jr * W2) * SB;
+                    in[u] = (i0 + SB <= F0) && (j0 + SB <= F1) && (k0 + SB <= F2);
+                    if (in[u]) {
+                        UNROLL for (int a = 0; a < SB; ++a) UNROLL for (int b = 0; b < SB; ++b) UNROLL for (int c = 0; c < SB; ++c) {
+                            const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
+                            xv[u][(a * SB + b) * SB + c] = V.im[fv];
+                            mv[u][(a * SB + b) * SB + c] = prep_mask<S, MK>(V, fv);
+                        }
                     }
                 }
             }
+            UNROLL for (int u = 0; u < UN; ++u) {
+                const int v = v0 + u * kBlock;
+                if (v >= n) break;
+                const int jr = (int)fdiv((uint32_t)v, V.mW2);
+                const int J = sl.y0 + jr, K = v - jr * W2;
+                int nm = 0, nb = 0;
+                float acc = 0.0f;
+                if (in[u]) {
+                    UNROLL for (int q = 0; q < NV; ++q) {
+                        ++nb;
+                        if (mv[u][q] && xv[u][q] > rp.low_f && isfinite(xv[u][q])) {
+                            ++nm;
+                            acc += logf(xv[u][q]);
+                        }
+                    }
+                } else {
+                    prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
+                }
+                prep_commit(V, x, J, K, nm, nb, acc, nfull, nwork, mn, mx);
+            }
         }
-        UNROLL for (int u = 0; u < UN; ++u) {
-            const int v = v0 + u * kBlock;
-            if (v >= n) break;
+    } else {
+        for (int v = t; v < sl.ny * W2; v += kBlock) {
             const int jr = (int)fdiv((uint32_t)v, V.mW2);
             const int J = sl.y0 + jr, K = v - jr * W2;
             int nm = 0, nb = 0;
             float acc = 0.0f;
-            if (in[u]) {
-                UNROLL for (int q = 0; q < NV; ++q) {
-                    ++nb;
-                    if (mv[u][q] && xv[u][q] > rp.low_f && isfinite(xv[u][q])) {
-                        ++nm;
-                        acc += logf(xv[u][q]);
-                    }
-                }
-            } else {
-                prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
-            }
-            const size_t w = ((size_t)x * W1 + J) * W2 + K;
-            const bool fg = nm > 0 && 2 * nm >= nb;
-            const float lg = nm > 0 ? acc / (float)nm : 0.0f;
-            V.imlog[w] = lg;
-            V.logU[w] = lg;
-            V.wmask[w] = fg ? 1 : 0;
-            nfull += nm;
-            if (fg) {
-                nwork += 1.0;
-                mn = fmin(mn, (double)lg);
-                mx = fmax(mx, (double)lg);
-            }
+            prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
+            prep_commit(V, x, J, K, nm, nb, acc, nfull, nwork, mn, mx);
         }
     }
     red[t] = nfull;
@@ -1438,13 +1449,14 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
                 const float* wz = S.zw + 4 * z;
                 const float* br = S.B + r * Cz + S.zb[z];
                 const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
-                // write-once outputs: streaming (nontemporal) stores
-                if (V.out_logbias) store_stream(V.out_logbias + base + v, f);
-                if (V.out_bias) store_stream(V.out_bias + base + v, expf(f));
-                if (V.out_norm) {
-                    // im / exp(f) as im * exp(-f): no division per voxel
-                    const float xo = (xin[u] <= rp.low_f) ? 0.0f : xin[u];  // N4.py:285
-                    store_stream(V.out_norm + base + v, xo * expf(-f));
+                if (V.out_logbias) V.out_logbias[base + v] = f;
+                if (V.out_norm || V.out_bias) {
+                    const float ef = expf(f);
+                    if (V.out_bias) V.out_bias[base + v] = ef;
+                    if (V.out_norm) {
+                        const float xo = (xin[u] <= rp.low_f) ? 0.0f : xin[u];  // N4.py:285
+                        V.out_norm[base + v] = xo / ef;
+                    }
                 }
             }
         }
