@@ -479,6 +479,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.ppart = (double*)(base + o.ppart);
         D.prep_rows = I.prep_rows;
         D.prep_tiles_x = I.prep_tiles_x;
+        D.recon_rows = I.recon_rows;
         D.wtile_rows = I.wtile_rows;
         D.wtiles_x = I.wtiles_x;
         D.hist_copies = hist_copies(I.W[0] * I.wtiles_x, tiled);

@@ -112,6 +112,7 @@ struct VolDesc {
     int32_t part_stride;       // floats per slab in part (max Cy*Cz)
     int32_t prep_rows;         // working rows per prep tile
     int32_t prep_tiles_x;      // prep tiles per working slab
+    int32_t recon_rows;        // full-resolution rows per reconstruction tile
     int32_t wtile_rows;        // working rows per tile of the step kernels
     int32_t wtiles_x;          // such tiles per working slab (spart has W0*wtiles_x entries)
     int32_t hist_copies;       // tile t adds its histogram into copy t % hist_copies

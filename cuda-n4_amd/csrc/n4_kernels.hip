@@ -1382,15 +1382,19 @@ struct ReconLds {
     float* B;
     float* zw;
     int* zb;
+    float* yw;  // [tile rows][4] y factors of the tile's rows
+    int* yb;
 };
 
-HD size_t carve_recon(unsigned char* base, int Z, int R, int Cy, int Cz, ReconLds* o) {
+HD size_t carve_recon(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, ReconLds* o) {
     Carve c{base, 0};
     ReconLds L;
     L.A = c.take<float>((size_t)Cy * Cz);
     L.B = c.take<float>((size_t)R * Cz);
     L.zw = c.take<float>((size_t)4 * Z);
     L.zb = c.take<int>(Z);
+    L.yw = c.take<float>((size_t)4 * Yt);
+    L.yb = c.take<int>(Yt);
     if (o) *o = L;
     return align16(c.off);
 }
@@ -1405,9 +1409,11 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
     const int CyCz = Cy * Cz;
     const int R = rows_per_chunk(Y, Z);
     ReconLds S;
-    carve_recon(smem, Z, R, Cy, Cz, &S);
+    carve_recon(smem, Z, R, V.recon_rows, Cy, Cz, &S);
     const int x = sl.x;
     const AxisEntry ex = V.fax[0][x];
+    // set-up: x-contracted planes, and the z and (tile) y factor tables, all
+    // from one round of global loads
     BLOCK_FOR(t)
     for (int pr = t; pr < CyCz; pr += kBlock) {
         float a = 0.0f;
@@ -1419,6 +1425,11 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
         for (int k = 0; k < 4; ++k) S.zw[4 * z + k] = e.w[k];
         S.zb[z] = e.b;
     }
+    for (int r = t; r < sl.ny; r += kBlock) {
+        const AxisEntry& e = V.fax[1][sl.y0 + r];
+        for (int k = 0; k < 4; ++k) S.yw[4 * r + k] = e.w[k];
+        S.yb[r] = e.b;
+    }
     BLOCK_END
     BLOCK_SYNC();
     const int yend = sl.y0 + sl.ny;
@@ -1428,9 +1439,11 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
-            const AxisEntry& e = V.fax[1][y0 + r];
+            const int tr = y0 - sl.y0 + r;  // tile row
+            const float* yw = S.yw + 4 * tr;
+            const int b0 = S.yb[tr];
             float b = 0.0f;
-            for (int k = 0; k < 4; ++k) b += e.w[k] * S.A[(e.b + k) * Cz + c];
+            for (int k = 0; k < 4; ++k) b += yw[k] * S.A[(b0 + k) * Cz + c];
             S.B[pr] = b;
         }
         BLOCK_END
@@ -1601,7 +1614,7 @@ size_t lds_eval(const VolDesc& v, int level, int prec) {
 
 size_t lds_recon(const VolDesc& v) {
     const LevelDesc& L = v.lev[v.nlevels - 1];
-    return carve_recon(nullptr, v.F[2], rows_per_chunk(v.F[1], v.F[2]), L.C[1], L.C[2], nullptr);
+    return carve_recon(nullptr, v.F[2], rows_per_chunk(v.F[1], v.F[2]), v.recon_rows, L.C[1], L.C[2], nullptr);
 }
 
 int mfma_tiles(int c) { return cp_tiles(c); }
