@@ -83,7 +83,7 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-# Algorithmic HBM bytes per unit (also stated in DESIGN.md):
+# Algorithmic HBM bytes per unit:
 #   working-grid kernels, per working voxel and iteration executed:
 #     hist 5 B (logU 4 + mask 1), fit 5 B (logU 4 + mask 1) plus 1 B (mask)
 #     per level for the level's denominator, eval 9 B (mask 1 + logU read 4 +

@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/n4_oracle.c).
 
 TEST INFRASTRUCTURE ONLY.  The oracle restates the reference's N4.py in FP64
-and is the parity checker for the HIP product path.  Only tests/,
+and is the parity reference for the HIP product path.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import it; the
 product (the HIP library and its host wrappers) never does.
 """

@@ -1,7 +1,7 @@
 /*
  * n4_oracle.c -- CPU FP64 restatement of the reference N4 (N4.py).
  *
- * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP product path.
+ * TEST INFRASTRUCTURE ONLY: the parity reference for the HIP product path.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  Every function cites the N4.py lines it restates; deviations are
  * listed in n4_oracle.h.  Built with -ffp-contract=off so that every FP64

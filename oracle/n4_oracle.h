@@ -1,7 +1,7 @@
 /*
  * n4_oracle.h -- CPU FP64 restatement of the reference N4 algorithm.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference for the HIP
  * product path: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  Nothing in the product links against it.
  *
