@@ -794,11 +794,17 @@ static n4_status launch_step_graph(n4_handle* h, int g, const GraphKey& key, F&&
 }
 #endif
 
+// Shrink classes of the prep kernels, in the order a group's volumes are
+// laid out (each class is one prep launch per group).
+static const int kPrepClasses[5] = {1, 2, 3, 4, 0};
+
 struct GroupRun {
     int v0 = 0, v1 = 0;    // volumes
     int ws0 = 0, ws1 = 0;  // working slabs
     int fs0 = 0, fs1 = 0;  // recon tiles
     int ps0 = 0, ps1 = 0;  // prep tiles
+    // prep tiles by shrink class (kPrepClasses order): [pc[i], pc[i + 1])
+    int pc[6] = {0, 0, 0, 0, 0, 0};
     void* stream = nullptr;
     int it = 0;            // steps enqueued
     bool finished = false;
@@ -841,10 +847,12 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             gid[e.second] = g;
             load[g] += -e.first;
         }
+        // within a group, volumes of one prep shrink class are contiguous
         int k = 0;
         for (int g = 0; g < G; ++g)
-            for (int v = 0; v < nv; ++v)
-                if (gid[v] == g) order[k++] = v;
+            for (int c : kPrepClasses)
+                for (int v = 0; v < nv; ++v)
+                    if (gid[v] == g && prep_class(in_user[v].s) == c) order[k++] = v;
     }
     std::vector<VolIn> in(nv);
     for (int k = 0; k < nv; ++k) in[k] = in_user[order[k]];
@@ -889,12 +897,16 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             R.ws0 = ws;
             R.fs0 = fs;
             R.ps0 = ps;
+            int ci = 0;
+            R.pc[0] = ps;
             while (v < nv && gid[order[v]] == g) {
+                while (ci < 4 && prep_class(P.desc[v].s) != kPrepClasses[ci]) R.pc[++ci] = ps;
                 ws += P.desc[v].W[0] * P.desc[v].wtiles_x;
                 fs += P.recon_tiles[v];
                 ps += P.desc[v].W[0] * P.desc[v].prep_tiles_x;
                 ++v;
             }
+            while (ci < 5) R.pc[++ci] = ps;
             R.v1 = v;
             R.ws1 = ws;
             R.fs1 = fs;
@@ -946,7 +958,10 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     for (int g = 0; g < G; ++g) {
         GroupRun& R = gr[g];
         void* gs = R.stream;
-        GLAUNCH(N4_K_PREP, launch_prep(dd, dev_pslab(h, P) + R.ps0, R.ps1 - R.ps0, 0, rp, gs));
+        for (int c = 0; c < 5; ++c)
+            if (R.pc[c + 1] > R.pc[c])
+                GLAUNCH(N4_K_PREP,
+                        launch_prep(dd, dev_pslab(h, P) + R.pc[c], R.pc[c + 1] - R.pc[c], kPrepClasses[c], rp, gs));
         GLAUNCH(N4_K_LEVEL, launch_level(dd + R.v0, R.v1 - R.v0, 0, rp, gs));
     }
     int remaining = G;

@@ -36,7 +36,11 @@ size_t lds_flat(int nbins);
 
 // `stream` is a hipStream_t (void* keeps this header HIP-free).  Return 0
 // on success, else a hipError_t value.
-int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink_hint, RunParams rp, void* stream);
+// Prep kernels are instantiated per shrink class: 1..4, and 0 for any
+// other factor (the class-0 kernel handles every factor).  All slabs of one
+// launch_prep call must belong to volumes of class prep_class(shrink).
+inline int prep_class(int shrink) { return (shrink >= 1 && shrink <= 4) ? shrink : 0; }
+int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink, RunParams rp, void* stream);
 // level 0: start of a correction (counts, cleared accumulators, state
 // TRANS(0)); level > 0: refinement of level-1's lattice (stage call).
 int launch_level(const VolDesc* vols, int nvol, int level, RunParams rp, void* stream);

@@ -107,6 +107,32 @@ HD void sincospi_d(double x, double* s, double* c) {
 
 HD size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
 
+// Global-memory view of a descriptor pointer.  A pointer loaded from a
+// descriptor is generic to the compiler, which then emits flat loads and
+// stores; those also count against the LDS wait counter, so every LDS read
+// after them waits for the outstanding memory traffic.  Accesses through
+// gp(p) are global_load / global_store.
+#ifdef N4_DEVICE_PASS
+#define N4_GLOBAL __attribute__((address_space(1)))
+#else
+#define N4_GLOBAL
+#endif
+template <class T>
+HD N4_GLOBAL T* gp(T* p) {
+    return (N4_GLOBAL T*)p;
+}
+
+// One 16-byte global load of a voxel quad (p 16-byte aligned).
+HD F4 load_quad(const float* p) {
+#ifdef N4_DEVICE_PASS
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f x = *reinterpret_cast<const N4_GLOBAL v4f*>(gp(p));
+    return F4{{x[0], x[1], x[2], x[3]}};
+#else
+    return *reinterpret_cast<const F4*>(p);
+#endif
+}
+
 // Index of a step-kernel tile within its volume (its slot in spart).
 HD int tile_id(const VolDesc& V, const Slab& sl) {
     return sl.x * V.wtiles_x + (V.wtiles_x > 1 ? sl.y0 / V.wtile_rows : 0);
@@ -261,8 +287,8 @@ HD int rows_per_chunk(int Y, int Z) {
 // S > 0: shrink factor known at compile time (unrolled); S == 0: any.
 template <int S, int MK>
 HD bool prep_mask(const VolDesc& V, size_t fv) {
-    if (MK == 0) return reinterpret_cast<const float*>(V.mask)[fv] != 0.0f;
-    if (MK == 1) return reinterpret_cast<const uint8_t*>(V.mask)[fv] != 0;
+    if (MK == 0) return gp(reinterpret_cast<const float*>(V.mask))[fv] != 0.0f;
+    if (MK == 1) return gp(reinterpret_cast<const uint8_t*>(V.mask))[fv] != 0;
     return true;
 }
 
@@ -280,7 +306,7 @@ HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, i
             UNROLL for (int b = 0; b < SS; ++b) {
                 UNROLL for (int c = 0; c < SS; ++c) {
                     const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
-                    xv[b * SS + c] = V.im[fv];
+                    xv[b * SS + c] = gp(V.im)[fv];
                     mv[b * SS + c] = prep_mask<S, MK>(V, fv);
                 }
             }
@@ -302,7 +328,7 @@ HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, i
             const size_t row = ((size_t)i * F1 + j) * F2;
             for (int k = k0; k < k1; ++k) {
                 ++nb;
-                const float xv = V.im[row + k];
+                const float xv = gp(V.im)[row + k];
                 if (prep_mask<S, MK>(V, row + k) && xv > rp.low_f && isfinite(xv)) {
                     ++nm;
                     acc += logf(xv);
@@ -318,9 +344,9 @@ HD void prep_commit(const VolDesc& V, int x, int J, int K, int nm, int nb, float
     const size_t w = ((size_t)x * V.W[1] + J) * V.W[2] + K;
     const bool fg = nm > 0 && 2 * nm >= nb;
     const float lg = nm > 0 ? acc / (float)nm : 0.0f;
-    V.imlog[w] = lg;
-    V.logU[w] = lg;
-    V.wmask[w] = fg ? 1 : 0;
+    gp(V.imlog)[w] = lg;
+    gp(V.logU)[w] = lg;
+    gp(V.wmask)[w] = fg ? 1 : 0;
     nfull += nm;
     if (fg) {
         nwork += 1.0;
@@ -364,7 +390,7 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsign
                     if (in[u]) {
                         UNROLL for (int a = 0; a < SB; ++a) UNROLL for (int b = 0; b < SB; ++b) UNROLL for (int c = 0; c < SB; ++c) {
                             const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
-                            xv[u][(a * SB + b) * SB + c] = V.im[fv];
+                            xv[u][(a * SB + b) * SB + c] = gp(V.im)[fv];
                             mv[u][(a * SB + b) * SB + c] = prep_mask<S, MK>(V, fv);
                         }
                     }
@@ -653,8 +679,8 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
             const int qi = i0 + u * kBlock + t;
             const size_t w = q0 + 4 * (size_t)qi;
             const bool ok = qi < nq;
-            ms[u] = ok ? *reinterpret_cast<const uint32_t*>(V.wmask + w) : 0u;
-            xs[u] = ok ? *reinterpret_cast<const F4*>(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
+            ms[u] = ok ? *reinterpret_cast<const N4_GLOBAL uint32_t*>(gp(V.wmask) + w) : 0u;
+            xs[u] = ok ? load_quad(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
         }
         UNROLL for (int u = 0; u < 4; ++u) {
             const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
@@ -1135,8 +1161,8 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
                 const int qi = i0 + u * kBlock + t;
                 const size_t w = q0 + 4 * (size_t)qi;
                 const bool ok = qi < nq;
-                ms[u] = ok ? *reinterpret_cast<const uint32_t*>(V.wmask + w) : 0u;
-                xs[u] = (ok && !den) ? *reinterpret_cast<const F4*>(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
+                ms[u] = ok ? *reinterpret_cast<const N4_GLOBAL uint32_t*>(gp(V.wmask) + w) : 0u;
+                xs[u] = (ok && !den) ? load_quad(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
             }
             UNROLL for (int u = 0; u < 4; ++u) {
                 const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
@@ -1335,8 +1361,8 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
             UNROLL for (int u = 0; u < 8; ++u) {
                 const int v = v0 + u * kBlock + t;
                 const bool ok = v < ny * Z;
-                ms[u] = ok ? V.wmask[base + v] : 0;
-                lu[u] = (ok && ms[u]) ? V.logU[base + v] : 0.0f;
+                ms[u] = ok ? gp(V.wmask)[base + v] : 0;
+                lu[u] = (ok && ms[u]) ? gp(V.logU)[base + v] : 0.0f;
             }
             UNROLL for (int u = 0; u < 8; ++u) {
                 if (!ms[u]) continue;
@@ -1346,7 +1372,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
                 const float* brr = S.Br + r * Cz + S.zb[z];
                 const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
                 const float lg = lu[u] - d;
-                V.logU[base + v] = lg;
+                gp(V.logU)[base + v] = lg;
                 const double uu = (double)expm1f(-d);
                 s1 += uu;
                 s2 += uu * uu;
@@ -1386,6 +1412,16 @@ struct ReconLds {
     int* yb;
 };
 
+// Rows per y-contracted chunk of a reconstruction tile: the whole tile when
+// its rows of Cz floats fit kReconChunkFloats, so a tile streams with no
+// barrier between its voxels.
+constexpr int kReconChunkFloats = 8192;
+HD int recon_chunk_rows(int tile_rows, int Cz) {
+    int r = kReconChunkFloats / (Cz > 0 ? Cz : 1);
+    if (r < 1) r = 1;
+    return r < tile_rows ? r : tile_rows;
+}
+
 HD size_t carve_recon(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, ReconLds* o) {
     Carve c{base, 0};
     ReconLds L;
@@ -1407,7 +1443,7 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
     const int Y = V.F[1], Z = V.F[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
     const int CyCz = Cy * Cz;
-    const int R = rows_per_chunk(Y, Z);
+    const int R = recon_chunk_rows(V.recon_rows, Cz);
     ReconLds S;
     carve_recon(smem, Z, R, V.recon_rows, Cy, Cz, &S);
     const int x = sl.x;
@@ -1449,26 +1485,33 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
         BLOCK_END
         BLOCK_SYNC();
         BLOCK_FOR(t)
-        for (int v0 = 0; v0 < ny * Z; v0 += 4 * kBlock) {
-            float xin[4];
-            UNROLL for (int u = 0; u < 4; ++u) {
+        const N4_GLOBAL float* im = gp(V.im);
+        N4_GLOBAL float* o_lb = gp(V.out_logbias);
+        N4_GLOBAL float* o_b = gp(V.out_bias);
+        N4_GLOBAL float* o_n = gp(V.out_norm);
+        // consecutive lanes on consecutive voxels, kReconLoads image loads
+        // in flight per thread
+        constexpr int kReconLoads = 16;
+        for (int v0 = 0; v0 < ny * Z; v0 += kReconLoads * kBlock) {
+            float xin[kReconLoads];
+            UNROLL for (int u = 0; u < kReconLoads; ++u) {
                 const int v = v0 + u * kBlock + t;
-                xin[u] = (V.out_norm && v < ny * Z) ? V.im[base + v] : 0.0f;
+                xin[u] = (o_n && v < ny * Z) ? im[base + v] : 0.0f;
             }
-            UNROLL for (int u = 0; u < 4; ++u) {
+            UNROLL for (int u = 0; u < kReconLoads; ++u) {
                 const int v = v0 + u * kBlock + t;
                 if (v >= ny * Z) continue;
                 const int r = (int)fdiv((uint32_t)v, V.mF2), z = v - r * Z;
                 const float* wz = S.zw + 4 * z;
                 const float* br = S.B + r * Cz + S.zb[z];
                 const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
-                if (V.out_logbias) V.out_logbias[base + v] = f;
-                if (V.out_norm || V.out_bias) {
+                if (o_lb) o_lb[base + v] = f;
+                if (o_n || o_b) {
                     const float ef = expf(f);
-                    if (V.out_bias) V.out_bias[base + v] = ef;
-                    if (V.out_norm) {
+                    if (o_b) o_b[base + v] = ef;
+                    if (o_n) {
                         const float xo = (xin[u] <= rp.low_f) ? 0.0f : xin[u];  // N4.py:285
-                        V.out_norm[base + v] = xo / ef;
+                        o_n[base + v] = xo / ef;
                     }
                 }
             }
@@ -1614,7 +1657,8 @@ size_t lds_eval(const VolDesc& v, int level, int prec) {
 
 size_t lds_recon(const VolDesc& v) {
     const LevelDesc& L = v.lev[v.nlevels - 1];
-    return carve_recon(nullptr, v.F[2], rows_per_chunk(v.F[1], v.F[2]), v.recon_rows, L.C[1], L.C[2], nullptr);
+    return carve_recon(nullptr, v.F[2], recon_chunk_rows(v.recon_rows, L.C[2]), v.recon_rows, L.C[1], L.C[2],
+                       nullptr);
 }
 
 int mfma_tiles(int c) { return cp_tiles(c); }
@@ -1624,67 +1668,72 @@ int mfma_tiles(int c) { return cp_tiles(c); }
 
 #define SMEM() reinterpret_cast<unsigned char*>(smem_)
 
-__global__ __launch_bounds__(kBlock) void k_prep(const VolDesc* vols, const Slab* slabs, RunParams rp) {
+// One instance per shrink factor 1..4 (S = 0: any factor), so each keeps
+// the register count of its own case.
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_prep(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs,
+                                                 RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
-    body_prep(vols, slabs, rp, blockIdx.x, SMEM());
+    const Slab sl = slabs[blockIdx.x];
+    body_prep_s<S>(vols[sl.vol], sl, rp, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_level(const VolDesc* vols, int level, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_level(const VolDesc* __restrict__ vols, int level, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_level(vols, level, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp,
+__global__ __launch_bounds__(kBlock) void k_hist(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int step, RunParams rp,
                                                  int32_t* dbg_idx) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_hist(vols, slabs, step, rp, dbg_idx, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_wiener(const VolDesc* vols, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_wiener(const VolDesc* __restrict__ vols, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_wiener(vols, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_fit(const VolDesc* vols, const Slab* slabs, int level, int it,
+__global__ __launch_bounds__(kBlock) void k_fit(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level, int it,
                                                 int mode, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_fit(vols, slabs, level, it, mode, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* vols, int level, int mode, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* __restrict__ vols, int level, int mode, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_lattice(vols, level, mode, rp, blockIdx.x, blockIdx.y, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_update(const VolDesc* vols, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_update(const VolDesc* __restrict__ vols, RunParams rp) {
     body_update(vols, rp, (int)blockIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* vols, const Slab* slabs, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_eval(vols, slabs, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_recon(const VolDesc* vols, const Slab* slabs, int level,
+__global__ __launch_bounds__(kBlock) void k_recon(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level,
                                                   RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_recon(vols, slabs, level, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_flat(const VolDesc* vols, const Slab* slabs, int op,
+__global__ __launch_bounds__(kBlock) void k_flat(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int op,
                                                  const float* a, const float* b, float* out, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_flat(vols, slabs, op, a, b, out, rp, blockIdx.x, SMEM());
 }
 
 template <int PREC, int MT, int NT>
-__global__ __launch_bounds__(kBlock) void k_fit_mfma(const VolDesc* vols, const Slab* slabs, int level, int it,
+__global__ __launch_bounds__(kBlock) void k_fit_mfma(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level, int it,
                                                      int mode, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_fit_mfma<PREC, MT, NT>(vols, slabs, level, it, mode, rp, blockIdx.x, SMEM());
 }
 
 template <int PREC, int MT, int NT>
-__global__ __launch_bounds__(kBlock) void k_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp) {
+__global__ __launch_bounds__(kBlock) void k_eval_mfma(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_eval_mfma<PREC, MT, NT>(vols, slabs, rp, blockIdx.x, SMEM());
 }
@@ -1719,9 +1768,17 @@ static bool mfma_ok(const RunParams& rp, int mt, int nt) {
 
 static int last_launch_status() { return (int)hipGetLastError(); }
 
-int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int, RunParams rp, void* stream) {
+int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink, RunParams rp, void* stream) {
     if (nslab <= 0) return 0;
-    hipLaunchKernelGGL(k_prep, dim3(nslab), dim3(kBlock), lds_prep(), (hipStream_t)stream, vols, slabs, rp);
+    const dim3 g(nslab), b(kBlock);
+    const hipStream_t s = (hipStream_t)stream;
+    switch (prep_class(shrink)) {
+        case 1: hipLaunchKernelGGL(k_prep<1>, g, b, lds_prep(), s, vols, slabs, rp); break;
+        case 2: hipLaunchKernelGGL(k_prep<2>, g, b, lds_prep(), s, vols, slabs, rp); break;
+        case 3: hipLaunchKernelGGL(k_prep<3>, g, b, lds_prep(), s, vols, slabs, rp); break;
+        case 4: hipLaunchKernelGGL(k_prep<4>, g, b, lds_prep(), s, vols, slabs, rp); break;
+        default: hipLaunchKernelGGL(k_prep<0>, g, b, lds_prep(), s, vols, slabs, rp); break;
+    }
     return last_launch_status();
 }
 
@@ -1805,8 +1862,10 @@ static unsigned char* emu_lds(size_t n) {
     return buf.data();
 }
 
-int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int, RunParams rp, void*) {
+int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink, RunParams rp, void*) {
     unsigned char* b = emu_lds(lds_prep());
+    for (int i = 0; i < nslab; ++i)
+        if (prep_class(vols[slabs[i].vol].s) != prep_class(shrink)) return 1;  // a launch of mixed classes
     for (int i = 0; i < nslab; ++i) body_prep(vols, slabs, rp, i, b);
     return 0;
 }

@@ -198,8 +198,8 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
         UNROLL for (int r = 0; r < 16; ++r) {
             const bool ok = z < Z && y0 + r < Yt;
             const size_t v = ((size_t)x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
-            ms[r] = ok ? V.wmask[v] : 0;
-            xs[r] = (ok && !den) ? V.logU[v] : 0.0f;
+            ms[r] = ok ? gp(V.wmask)[v] : 0;
+            xs[r] = (ok && !den) ? gp(V.logU)[v] : 0.0f;
         }
     };
     load_rows(16 * wv, 0);
@@ -429,13 +429,13 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
             float lu[4];
             UNROLL for (int i = 0; i < 4; ++i) {
                 const bool ok = yok && (zb + i) < Z;
-                ms[i] = ok ? V.wmask[row + zb + i] : 0;
-                lu[i] = (ok && ms[i]) ? V.logU[row + zb + i] : 0.0f;
+                ms[i] = ok ? gp(V.wmask)[row + zb + i] : 0;
+                lu[i] = (ok && ms[i]) ? gp(V.logU)[row + zb + i] : 0.0f;
             }
             UNROLL for (int i = 0; i < 4; ++i) {
                 if (!ms[i]) continue;
                 const float lg = lu[i] - d[i];
-                V.logU[row + zb + i] = lg;
+                gp(V.logU)[row + zb + i] = lg;
                 const double uu = (double)expm1f(-d[i]);
                 s1 += uu;
                 s2 += uu * uu;
