@@ -1288,16 +1288,20 @@ struct EvalLds {
     float* Br;
     float* zw;
     int* zb;
+    float* yw;  // [tile rows][4] y factors of the tile's rows
+    int* yb;
     double* red;
 };
 
-HD size_t carve_eval(unsigned char* base, int Z, int R, int Cy, int Cz, EvalLds* o) {
+HD size_t carve_eval(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, EvalLds* o) {
     Carve c{base, 0};
     EvalLds L;
     L.Ar = c.take<float>((size_t)Cy * Cz);
     L.Br = c.take<float>((size_t)R * Cz);
     L.zw = c.take<float>((size_t)4 * Z);
     L.zb = c.take<int>(Z);
+    L.yw = c.take<float>((size_t)4 * Yt);
+    L.yb = c.take<int>(Yt);
     L.red = c.take<double>((size_t)4 * kBlock);
     if (o) *o = L;
     return align16(c.off);
@@ -1314,7 +1318,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     const int CyCz = Cy * Cz;
     const int R = rows_per_eval_chunk(Y, Z);
     EvalLds S;
-    carve_eval(smem, Z, R, Cy, Cz, &S);
+    carve_eval(smem, Z, R, V.wtile_rows, Cy, Cz, &S);
     const int x = sl.x;
     const AxisEntry ex = LD.ax[0][x];
     const float* LR = V.lresf + (size_t)ex.b * CyCz;  // the 4 increment planes slab x touches
@@ -1329,6 +1333,12 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
         for (int k = 0; k < 4; ++k) S.zw[4 * z + k] = e.w[k];
         S.zb[z] = e.b;
     }
+    // y factors of the tile's rows, in the same round of loads
+    for (int r = t; r < sl.ny; r += kBlock) {
+        const AxisEntry& e = LD.ax[1][sl.y0 + r];
+        for (int k = 0; k < 4; ++k) S.yw[4 * r + k] = e.w[k];
+        S.yb[r] = e.b;
+    }
     S.red[t] = 0.0;
     S.red[kBlock + t] = 0.0;
     S.red[2 * kBlock + t] = INFINITY;
@@ -1342,9 +1352,11 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
         BLOCK_FOR(t)
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
-            const AxisEntry& e = LD.ax[1][y0 + r];
+            const int tr = y0 - sl.y0 + r;  // tile row
+            const float* yw = S.yw + 4 * tr;
+            const int b0 = S.yb[tr];
             float br = 0.0f;
-            for (int k = 0; k < 4; ++k) br += e.w[k] * S.Ar[(e.b + k) * Cz + c];
+            for (int k = 0; k < 4; ++k) br += yw[k] * S.Ar[(b0 + k) * Cz + c];
             S.Br[pr] = br;
         }
         BLOCK_END
@@ -1354,17 +1366,19 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
         // min/max of FP32 values are exact in FP32
         float mn = (float)S.red[2 * kBlock + t], mx = (float)S.red[3 * kBlock + t];
         // consecutive lanes on consecutive voxels (conflict-free LDS
-        // reads of the z tables), eight loads in flight per thread
-        for (int v0 = 0; v0 < ny * Z; v0 += 8 * kBlock) {
-            uint8_t ms[8];
-            float lu[8];
-            UNROLL for (int u = 0; u < 8; ++u) {
+        // reads of the z tables), kEvalLoads mask and logU loads in flight
+        // per thread, all independent (logU is read whatever the mask)
+        constexpr int kEvalLoads = 8;
+        for (int v0 = 0; v0 < ny * Z; v0 += kEvalLoads * kBlock) {
+            uint8_t ms[kEvalLoads];
+            float lu[kEvalLoads];
+            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
                 const int v = v0 + u * kBlock + t;
                 const bool ok = v < ny * Z;
                 ms[u] = ok ? gp(V.wmask)[base + v] : 0;
-                lu[u] = (ok && ms[u]) ? gp(V.logU)[base + v] : 0.0f;
+                lu[u] = ok ? gp(V.logU)[base + v] : 0.0f;
             }
-            UNROLL for (int u = 0; u < 8; ++u) {
+            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
                 if (!ms[u]) continue;
                 const int v = v0 + u * kBlock + t;
                 const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
@@ -1652,7 +1666,7 @@ size_t lds_eval(const VolDesc& v, int level, int prec) {
                                prec, nullptr);
 #endif
     (void)prec;
-    return carve_eval(nullptr, v.W[2], rows_per_eval_chunk(v.W[1], v.W[2]), L.C[1], L.C[2], nullptr);
+    return carve_eval(nullptr, v.W[2], rows_per_eval_chunk(v.W[1], v.W[2]), v.wtile_rows, L.C[1], L.C[2], nullptr);
 }
 
 size_t lds_recon(const VolDesc& v) {

@@ -430,7 +430,7 @@ __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams
             UNROLL for (int i = 0; i < 4; ++i) {
                 const bool ok = yok && (zb + i) < Z;
                 ms[i] = ok ? gp(V.wmask)[row + zb + i] : 0;
-                lu[i] = (ok && ms[i]) ? gp(V.logU)[row + zb + i] : 0.0f;
+                lu[i] = ok ? gp(V.logU)[row + zb + i] : 0.0f;
             }
             UNROLL for (int i = 0; i < 4; ++i) {
                 if (!ms[i]) continue;
