@@ -1739,8 +1739,10 @@ __global__ __launch_bounds__(kBlock) void k_flat(const VolDesc* __restrict__ vol
     body_flat(vols, slabs, op, a, b, out, rp, blockIdx.x, SMEM());
 }
 
+// The one-tile instance (up to 16 control points along y and z) is held to
+// 96 VGPRs, so 5 workgroups fit a CU (its LDS limit) instead of 4.
 template <int PREC, int MT, int NT>
-__global__ __launch_bounds__(kBlock) void k_fit_mfma(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level, int it,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT * NT == 1 ? 5 : 1))) void k_fit_mfma(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level, int it,
                                                      int mode, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_fit_mfma<PREC, MT, NT>(vols, slabs, level, it, mode, rp, blockIdx.x, SMEM());
