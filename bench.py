@@ -127,6 +127,34 @@ def oracle_sample(vols, cfg, shrinks, cpu_seconds, nthreads):
     return out, t_cpu
 
 
+def _oracle_one(job):
+    import oracle as O
+    im, mask, levels, iters, shrink = job
+    O.n4(im, mask, levels=levels, max_iter=[iters] * levels, conv_threshold=0.001, shrink=shrink, nthreads=1)
+
+
+def oracle_all_cores(vols, cfg, shrinks, n):
+    """The same first n volumes through the oracle, one volume per process on
+    all host cores (at most 16): wall seconds and the worker count."""
+    from concurrent.futures import ProcessPoolExecutor
+    workers = max(1, min(16, os.cpu_count() or 1, n))
+    jobs = [(vols[i][0], vols[i][1], cfg["levels"], cfg["max_iterations"], shrinks[i]) for i in range(n)]
+    t1 = time.perf_counter()
+    with ProcessPoolExecutor(workers) as ex:
+        list(ex.map(_oracle_one, jobs))
+    return time.perf_counter() - t1, workers
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +193,9 @@ def main():
         cpu_threads = 1 if len(vols) > 1 else min(16, os.cpu_count() or 1)
         sample, t_cpu = (([], 0.0) if args.no_cpu_baseline
                          else oracle_sample(vols, cfg, shrinks, args.cpu_seconds, cpu_threads))
+        # the batch's sample again with one volume per host core
+        t_all, all_workers = (oracle_all_cores(vols, cfg, shrinks, len(sample))
+                              if sample and len(vols) > 1 else (0.0, 0))
     lib = n4amd.load()
     log(f"library {lib.version()} ({lib.path})")
     if args.copy_calibration_mib:
@@ -404,7 +435,12 @@ def main():
             "value": round(len(sample) / t_cpu, 4), "unit": "volumes/s", "cores": cpu_threads, "kind": "port",
             "sample": f"first {len(sample)} volume(s) of the workload, FP64 C restatement of N4.py (oracle/), "
                       f"{cpu_threads} thread(s), same levels/iterations/threshold/shrink",
+            "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
         }
+        if t_all > 0:
+            result["cpu_baseline"]["all_cores"] = {
+                "value": round(len(sample) / t_all, 4), "unit": "volumes/s", "cores": all_workers,
+                "sample": "the same volumes, one per process (one thread each) on all host cores (at most 16)"}
     print(json.dumps(result), flush=True)
 
 
