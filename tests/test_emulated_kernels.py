@@ -148,3 +148,22 @@ def test_mixed_shrink_batch(emu):
         ro = O.n4(im, m, levels=2, max_iter=[4, 4], conv_threshold=0.0, shrink=s)
         mm = (np.asarray(m) != 0) & (im > 10)
         assert np.max(np.abs(r["log_bias"].astype(np.float64) - ro["log_bias"])[mm]) <= 1e-4
+
+
+def test_prep_shrink_classes(emu):
+    """Volumes of every prep shrink class (1, 2 and the generic kernel's 5 and
+    6) in one batch over two stream groups: each volume gets the oracle's
+    result at its own shrink (the emulated launch also checks that a prep
+    launch never mixes classes)."""
+    import oracle as O
+    dims, shr = [(20, 18, 16), (41, 37, 33), (30, 26, 22), (22, 19, 17), (45, 40, 38)], [1, 5, 2, 1, 6]
+    vols = [PC.small_phantom(d, seed=120 + k)[:2] for k, d in enumerate(dims)]
+    emu.set_tuning(groups=2)
+    try:
+        out = emu.correct_batch(vols, levels=2, max_iter=[3, 3], fixed_iterations=True, shrink=shr)
+    finally:
+        emu.set_tuning(reset=True)
+    for (im, m), s, r in zip(vols, shr, out):
+        ro = O.n4(im, m, levels=2, max_iter=[3, 3], conv_threshold=0.0, shrink=s)
+        mm = (np.asarray(m) != 0) & (im > 10)
+        assert np.max(np.abs(r["log_bias"].astype(np.float64) - ro["log_bias"])[mm]) <= 1e-4, s

@@ -164,3 +164,24 @@ def test_n4_row_tiles(gpu, monkeypatch, precision):
     im, m, _ = PC.small_phantom((75, 62, 54), seed=9)
     print(f"{precision}: 16-row tiles on 75x62x54", flush=True)
     PC.check_n4_vs_oracle(gpu, im, m, 3, [10] * 3, 0.0, fixed=True, precision=precision)
+
+
+def test_prep_shrink_classes_batch(gpu):
+    """One batch over two stream groups holding every prep shrink class: 1,
+    2, 4 and the generic kernel's 5 and 7.  Each volume matches the oracle at
+    its own shrink."""
+    dims = [(40, 36, 32), (70, 64, 60), (52, 48, 40), (36, 34, 30), (80, 72, 66), (64, 60, 56)]
+    shr = [1, 5, 2, 1, 7, 4]
+    vols = [PC.small_phantom(d, seed=140 + k)[:2] for k, d in enumerate(dims)]
+    print("FP32: batch of shrink classes 1/2/4/generic over 2 groups", flush=True)
+    gpu.set_tuning(groups=2)
+    try:
+        out = gpu.correct_batch(vols, levels=2, max_iter=[6, 6], fixed_iterations=True, shrink=shr)
+    finally:
+        gpu.set_tuning(reset=True)
+    for (im, m), s, r in zip(vols, shr, out):
+        ro = O.n4(im, m, levels=2, max_iter=[6, 6], conv_threshold=0.0, shrink=s)
+        mm = (np.asarray(m) != 0) & (im > 10)
+        err = np.abs(r["log_bias"].astype(np.float64) - ro["log_bias"])[mm]
+        print(f"shrink {s}: max err {err.max():.3g}", flush=True)
+        assert err.max() <= 1e-4, s
