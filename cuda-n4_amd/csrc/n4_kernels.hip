@@ -1016,6 +1016,7 @@ struct FitLds {
     float* T1;
     float* T2;
     float* ywc;  // [Cy][ymax]  factor of control point cy at row y = lo(cy) + i
+    AxisEntry* ex;  // the slab's x factors (copied in the set-up round)
 };
 
 // Longest support (in grid points) of one cubic B-spline basis function over
@@ -1086,6 +1087,7 @@ HD size_t carve_fit(unsigned char* base, int nbins, int Y, int Z, int R, int Cy,
     L.T1 = c.take<float>((size_t)R * Cz);
     L.T2 = c.take<float>((size_t)Cy * Cz);
     L.ywc = c.take<float>((size_t)Cy * support_len(Y, Cy));
+    L.ex = c.take<AxisEntry>(1);
     if (o) *o = L;
     return align16(c.off);
 }
@@ -1144,6 +1146,9 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     for (int i = t; i < 2 * Cz; i += kBlock) S.zs[i] = LD.sup[2][i];
     for (int i = t; i < 2 * Cy; i += kBlock) S.ys[i] = LD.sup[1][i];
     for (int i = t; i < Cy * Cz; i += kBlock) S.T2[i] = 0.0f;
+    // the slab's x factors for the final scatter, in this round of loads
+    if (t < (int)(sizeof(AxisEntry) / 4))
+        reinterpret_cast<int32_t*>(S.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][x])[t];
     BLOCK_END
     BLOCK_SYNC();
     const int yend = sl.y0 + sl.ny;
@@ -1203,8 +1208,8 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         BLOCK_END
         BLOCK_SYNC();
     }
-    const AxisEntry ex = LD.ax[0][x];
     BLOCK_FOR(t)
+    const AxisEntry ex = *S.ex;
     for (int pr = t; pr < Cy * Cz; pr += kBlock) {
         if (scatter)
             scatter_numerator(V, LD, ex, tile_id(V, sl) % V.num_copies, den, pr, S.T2[pr]);

@@ -138,6 +138,7 @@ struct FitMfmaLds {
     float* red;
     unsigned short* Gh;  // [4 waves][16][kGStride]  staged g, BF16 hi
     unsigned short* Gl;  // same, lo (PREC 2)
+    AxisEntry* ex;       // the slab's x factors (copied in the set-up round)
 };
 
 // g staging: a wave loads 16 rows x 64 z with consecutive lanes on
@@ -159,6 +160,7 @@ HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, i
     L.red = c.take<float>((size_t)MT * NT * 256);
     L.Gh = c.take<unsigned short>((size_t)4 * 16 * kGStride);
     L.Gl = prec == 2 ? c.take<unsigned short>((size_t)4 * 16 * kGStride) : nullptr;
+    L.ex = c.take<AxisEntry>(1);
     if (o) *o = L;
     return align16(c.off);
 }
@@ -227,6 +229,9 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
         }
         S.Py.put(c, y, v);
     }
+    // the slab's x factors for the final scatter, in this round of loads
+    if (t < (int)(sizeof(AxisEntry) / 4))
+        reinterpret_cast<int32_t*>(S.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][x])[t];
     __syncthreads();
 
     f32x4 acc[MT][NT];
@@ -290,7 +295,7 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
         __syncthreads();
     }
     float* out = V.part + (size_t)x * V.part_stride;
-    const AxisEntry ex = LD.ax[0][x];
+    const AxisEntry ex = *S.ex;
     for (int e = t; e < MT * NT * 256; e += kBlock) {
         const int ln = e & 63, i = (e >> 6) & 3, tile = e >> 8;
         const int m = tile / NT, n = tile - (tile / NT) * NT;
