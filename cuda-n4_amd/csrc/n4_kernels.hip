@@ -613,8 +613,8 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
         }
         return;
     }
+    const double n = (double)st->n_mask_work;  // loaded with the state, before the reduction's barriers
     reduce_partials(V.spart, V.W[0] * V.wtiles_x, red);
-    const double n = (double)st->n_mask_work;
     const double mn = red[2 * kBlock], mx = red[3 * kBlock];
     double cvv = 0.0;
     int kind = kStepTrans, s_lvl = lvl;
@@ -819,14 +819,13 @@ HD D2* wave_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
 // N4.py:31-101 (cudaN4.cu:733-820 did this on the host with FFTW).  Two real
 // transforms share one complex FFT (V in the real part, F in the
 // imaginary).  Reads and clears the volume's histogram.
-HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigned char* smem) {
+HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, unsigned char* smem) {
     const int P = rp.P, half = P / 2, off = rp.offset, nb = rp.nbins;
     Carve cv{smem, 0};
     D2* z = cv.take<D2>(P);
     D2* y = cv.take<D2>(P);
     D2* ff = cv.take<D2>(P);
     D2* tw = cv.take<D2>(P);
-    const double bmin = st->bin_min, slope = st->slope;
     const double scaledFWHM = rp.fwhm / slope;
     const double expF = 4.0 * log(2.0) / (scaledFWHM * scaledFWHM);
     const double scale = 2.0 * sqrt(log(2.0) / M_PI) / scaledFWHM;
@@ -905,13 +904,15 @@ HD void wiener_table(const VolDesc& V, const VolState* st, RunParams rp, unsigne
 HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* smem) {
     const VolDesc& V = vols[bid];
     VolState* st = V.st;
-    const int kind = st->s_kind;
+    // the state fields of the step, in one round of loads
+    const int kind = st->s_kind, degenerate = st->degenerate, s_it = st->s_it;
+    const double bmin = st->bin_min, slope = st->slope;
     if (kind == kStepIter) {
-        if (!st->degenerate) wiener_table(V, st, rp, smem);
+        if (!degenerate) wiener_table(V, bmin, slope, rp, smem);
         BLOCK_FOR(t)
         if (t == 0) {
             st->mode = kModeIter;
-            st->it = st->s_it + 1;
+            st->it = s_it + 1;
         }
         BLOCK_END
     } else if (kind == kStepTrans) {
