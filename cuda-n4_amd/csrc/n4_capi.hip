@@ -255,7 +255,7 @@ static uint64_t magic_div(uint32_t d) { return ((uint64_t)1 << kDivShift) / d + 
 
 // Working voxels per prep tile and full-resolution voxels per recon tile:
 // enough workgroups to fill the device even for one volume.
-static const int kPrepTileVox = 1024;
+static const int kPrepTileVox = 512;
 // Default working voxels per tile of the step kernels.
 static const int kStepTileVox = 8192;
 static const int kReconTileVox = 8192;
