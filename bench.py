@@ -37,32 +37,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _gen(spec):
-    from n4amd import phantom as P
-    im, mask, _ = P.make_batch_volume(spec)
-    return im, mask
-
-
-def make_workload(name):
+def make_workload(name, mask_dtype=np.float32):
     from n4amd import phantom as P
     if name == "batch64":
-        specs = P.batch_specs(64, 128, 288, seed=0)
-        from concurrent.futures import ProcessPoolExecutor
-        workers = min(8, os.cpu_count() or 1)
-        with ProcessPoolExecutor(workers) as ex:
-            vols = list(ex.map(_gen, specs))
+        vols = P.generate_batch(P.batch_specs(64, 128, 288, seed=0), workers=min(16, os.cpu_count() or 1),
+                                mask_dtype=mask_dtype)
         cfg = {"workload": "batch of 64 head-class phantoms, axes U{128..288} (seed 0), mask 30-60%, "
                            "bias lattice 4^3-6^3 N(0,0.3), Rician sigma U[2,8], shrink 4, "
                            "4 levels x <=50 iterations, threshold 0.001",
                "volumes": 64, "shrink": 4, "levels": 4, "max_iterations": 50}
         return vols, cfg, [4] * 64
     if name == "batch256":
-        specs = P.batch_specs(256, 96, 320, seed=4)
+        vols = P.generate_batch(P.batch_specs(256, 96, 320, seed=4), workers=min(16, os.cpu_count() or 1),
+                                mask_dtype=mask_dtype)
         shrinks = P.batch_shrinks(256, seed=5)
-        from concurrent.futures import ProcessPoolExecutor
-        workers = min(8, os.cpu_count() or 1)
-        with ProcessPoolExecutor(workers) as ex:
-            vols = list(ex.map(_gen, specs))
         cfg = {"workload": "batch of 256 head-class phantoms, axes U{96..320} (seed 4), mask 30-60%, "
                            "bias lattice 4^3-6^3 N(0,0.3), Rician sigma U[2,8], shrink U{2,3,4} per volume "
                            "(seed 5), 4 levels x <=50 iterations, threshold 0.001",
@@ -145,6 +133,104 @@ def oracle_all_cores(vols, cfg, shrinks, n):
     return time.perf_counter() - t1, workers
 
 
+def config_record(lib, name, precs, steps):
+    """Compact record of one other workload: batched volumes/s per precision
+    (library default schedule, 1 untimed + median of max(5, steps) timed
+    runs), the plain single-volume FP32 rate (median of 5 passes) and the
+    log-bias error against the FP64 oracle (16 threads) on a stated sample."""
+    import n4amd
+    import oracle as O
+    t0 = time.time()
+    vols, cfg, shrinks = make_workload(name, mask_dtype=np.uint8)
+    vols = [(im, (np.asarray(m) != 0).astype(np.uint8)) for im, m in vols]
+    log(f"[{name}] generated {len(vols)} volume(s) in {time.time() - t0:.1f}s")
+    nvol = len(vols)
+    if nvol == 1:
+        sample = [0]
+        sample_text = "the volume"
+    else:  # the first volume of each shrink class, in index order
+        sample = sorted({s: i for i, s in reversed(list(enumerate(shrinks)))}.values())
+        sample_text = "first volume of each shrink class: " + ", ".join(f"#{i} (shrink {shrinks[i]})"
+                                                                         for i in sample)
+    levels, iters = cfg["levels"], cfg["max_iterations"]
+    ref = {}
+    for i in sample:
+        im, m = vols[i]
+        t1 = time.perf_counter()
+        ref[i] = O.n4(im, m, levels=levels, max_iter=[iters] * levels, conv_threshold=0.001, shrink=shrinks[i],
+                      nthreads=min(16, os.cpu_count() or 1))
+        log(f"[{name}] oracle volume {i}: {time.perf_counter() - t1:.1f}s, iterations {ref[i]['iterations']}")
+    kw = dict(levels=levels, max_iter=iters, threshold=0.001)
+    nc = n4amd.final_ncpt(n4amd.default_params(**kw))
+    arr = (n4amd.Volume * nvol)()
+    dev = []
+    for i, (im, m) in enumerate(vols):
+        d = (lib.to_device(im), lib.to_device(m), lib.empty(im.nbytes), lib.empty(4 * nc[0] * nc[1] * nc[2]),
+             lib.empty(im.nbytes) if i in ref else None)
+        dev.append(d)
+        v = arr[i]
+        v.d_im, v.d_mask, v.mask_kind = d[0].ptr, d[1].ptr, n4amd.MASK_U8
+        for k in range(3):
+            v.dims[k] = im.shape[k]
+        v.shrink_factor = shrinks[i]
+        v.d_im_normalized, v.d_lattice, v.d_bias_field = d[2].ptr, d[3].ptr, None
+        v.d_log_bias_field = d[4].ptr if d[4] is not None else None
+    n_work = [int(np.prod([(s + k - 1) // k for s, k in zip(im.shape, [shrinks[i]] * 3)]))
+              for i, (im, _) in enumerate(vols)]
+    reps = (n4amd.Report * nvol)()
+    out = {"workload": cfg["workload"], "volumes": nvol, "error_sample": sample_text, "batched": {}}
+    lib.set_tuning(reset=True)
+    for prec in precs:
+        p = n4amd.default_params(precision=prec, **kw)
+        lib._check(lib.L.n4_correct_batch(lib.h, arr, nvol, C.byref(p), reps, None))
+        ts = []
+        for _ in range(max(5, steps)):
+            t1 = time.perf_counter()
+            lib._check(lib.L.n4_correct_batch(lib.h, arr, nvol, C.byref(p), reps, None))
+            ts.append(time.perf_counter() - t1)
+        t = float(np.median(ts))
+        its = [[int(r.iterations[l]) for l in range(levels)] for r in reps]
+        errs = []
+        for i in sample:
+            im, m = vols[i]
+            lb = dev[i][4].download(np.float32, im.shape).astype(np.float64)
+            mm = (m != 0) & (im > 10)
+            e = np.abs(lb - ref[i]["log_bias"])[mm]
+            errs.append({"volume": i, "shrink": shrinks[i], "max": float(e.max()),
+                         "rms": float(np.sqrt(np.mean(e ** 2))),
+                         "iterations": its[i], "oracle_iterations": ref[i]["iterations"]})
+        out["batched"][prec] = {
+            "volumes_per_s": round(nvol / t, 3), "ms_median": round(1e3 * t, 4),
+            "mvoxel_iter_per_s": round(sum(w * sum(i) for w, i in zip(n_work, its)) / t / 1e6, 2),
+            "logbias_err_vs_fp64": {"max": max(e["max"] for e in errs), "rms_max": max(e["rms"] for e in errs),
+                                    "iteration_counts_matched":
+                                        f"{sum(e['iterations'] == e['oracle_iterations'] for e in errs)}/{len(errs)}",
+                                    "per_volume": errs}}
+        log(f"[{name}] {prec}: {nvol / t:.2f} volumes/s, err max {out['batched'][prec]['logbias_err_vs_fp64']['max']:.3g}")
+    p32 = n4amd.default_params(precision="fp32", **kw)
+    rep1 = n4amd.Report()
+
+    def plain_pass():
+        for i in range(nvol):
+            v = arr[i]
+            p32.shrink_factor = shrinks[i]
+            lib._check(lib.L.n4_correct(lib.h, v.d_im, v.d_mask, v.mask_kind, v.dims[0], v.dims[1], v.dims[2],
+                                        C.byref(p32), v.d_im_normalized, v.d_lattice, None, v.d_log_bias_field,
+                                        C.byref(rep1), None))
+    plain_pass()
+    lib.synchronize()
+    ts = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        plain_pass()
+        lib.synchronize()
+        ts.append(time.perf_counter() - t1)
+    t = float(np.median(ts))
+    out["plain_fp32"] = {"volumes_per_s": round(nvol / t, 3), "protocol": "1 untimed pass, median of 5"}
+    log(f"[{name}] plain FP32: {nvol / t:.2f} volumes/s")
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -168,6 +254,9 @@ def main():
     ap.add_argument("--schedule", default="autotune", choices=["autotune", "default", "direct"],
                     help="launch schedule of the batched runs: the library's autotuner (per precision, before "
                          "the warmup), its defaults, or its defaults without step graphs")
+    ap.add_argument("--configs", default="c1,c2,batch256",
+                    help="other workloads reported as compact sub-records (comma list of c1, c2, batch256; "
+                         "empty: none): volumes/s per precision, plain FP32 rate, log-bias error vs the oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -270,9 +359,12 @@ def main():
         for _ in range(args.warmup):
             batch_step(p)
         lib.synchronize()
+        step_s = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            batch_step(p)
+            t1 = time.perf_counter()
+            batch_step(p)  # synchronous: returns after the batch completes
+            step_s.append(time.perf_counter() - t1)
         lib.synchronize()
         dt = time.perf_counter() - t0
         # kernel durations, measured apart from the timed region with one
@@ -306,7 +398,8 @@ def main():
                                            for l in range(cfg["levels"])],
              "dominant": dominant, "dom_ms": dom_ms, "dom_n": dom_n, "dom_bytes": dom_bytes, "achieved": achieved,
              "breakdown": {k: round(v[0], 4) for k, v in breakdown.items()},
-             "err": sample_errors(its), "schedule": schedule, "autotune_s": round(t_tune, 2)}
+             "err": sample_errors(its), "schedule": schedule, "autotune_s": round(t_tune, 2),
+             "step_ms_median": 1e3 * float(np.median(step_s))}
         runs[prec] = R
         log(f"[{prec}] batched: {R['ms_per_step']:.3f} ms/step, {R['volumes_per_s']:.1f} volumes/s; dominant "
             f"{dominant}: {dom_ms / max(1, dom_n) * 1e3:.2f} us avg over {dom_n} launches, {achieved:.1f} GB/s; "
@@ -378,6 +471,7 @@ def main():
         result["roofline"]["achieved_at_rocprof_duration"] = round(bytes_per_launch / (prof_us * 1e-6) / 1e9, 2)
     result["kernel_ms_per_step_one_group"] = H["breakdown"]
     result["batched"] = {q: {"volumes_per_s": round(R["volumes_per_s"], 3), "ms_per_step": round(R["ms_per_step"], 4),
+                             "ms_per_step_median": round(R["step_ms_median"], 4),
                              "mvoxel_iter_per_s": round(R["mvoxel_iter_per_s"], 2),
                              "logbias_err_vs_fp64": R["err"], "kernel_ms_per_step": R["breakdown"],
                              "schedule": R["schedule"], "autotune_s": R["autotune_s"]}
@@ -422,13 +516,26 @@ def main():
                                             v.d_log_bias_field, C.byref(rep1), None))
         plain_pass()
         lib.synchronize()
-        t0 = time.perf_counter()
-        plain_pass()
-        lib.synchronize()
-        pdt = time.perf_counter() - t0
+        passes = []
+        for _ in range(max(5, args.steps)):
+            t0 = time.perf_counter()
+            plain_pass()
+            lib.synchronize()
+            passes.append(time.perf_counter() - t0)
+        pdt = float(np.median(passes))
         result["plain_fp32"] = {"volumes_per_s": round(nvol / pdt, 3), "ms_per_volume": round(1e3 * pdt / nvol, 4),
-                                "batched_over_plain": round(value / (nvol / pdt), 3)}
-        log(f"plain single-volume path: {nvol / pdt:.1f} volumes/s")
+                                "batched_over_plain": round(value / (nvol / pdt), 3),
+                                "protocol": f"1 untimed pass, median of {len(passes)} timed passes over the "
+                                            f"{nvol} volume(s)",
+                                "pass_ms": [round(1e3 * t, 3) for t in passes]}
+        log(f"plain single-volume path: {nvol / pdt:.1f} volumes/s (median of {len(passes)})")
+
+    if args.configs:
+        del dev, arr
+        result["configs"] = {}
+        for name in [c for c in args.configs.split(",") if c]:
+            result["configs"][name] = config_record(lib, name, precs, args.steps)
+            lib.set_tuning(reset=True)
 
     if sample:
         result["cpu_baseline"] = {

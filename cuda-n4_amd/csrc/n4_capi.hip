@@ -1000,6 +1000,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
 #ifndef N4_EMULATE
                     GraphKey key;
                     key.add(dd);
+                    key.add(gv);  // the group's first volume (k_wiener, k_update)
                     key.add(gsl);
                     key.add(nws);
                     key.add(nvg);

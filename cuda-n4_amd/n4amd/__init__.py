@@ -366,38 +366,53 @@ class Library:
             out["log_bias"] = d_lb.download(np.float32, dims)
         return out
 
-    def correct_batch(self, vols, params: Params | None = None, want_log_bias=True, shrink=None, **kw):
-        """vols: list of (im, mask) host arrays (mask may be None)."""
+    def correct_batch(self, vols, params: Params | None = None, want_log_bias=True, shrink=None, outputs=None,
+                      **kw):
+        """vols: list of (im, mask) host arrays (mask may be None; a uint8
+        mask goes to the device as N4_MASK_U8, any other dtype as float32).
+        outputs: indices of the volumes whose arrays are downloaded and
+        whose log bias is computed (None: all); the others report their
+        iteration counts only."""
         p = params if params is not None else default_params(**kw)
         nc = final_ncpt(p)
-        keep, arr = self._batch_arrays(vols, p, want_log_bias, shrink)
+        sel = None if outputs is None else set(int(i) for i in outputs)
+        keep, arr = self._batch_arrays(vols, p, want_log_bias, shrink, sel)
         reps = (Report * len(vols))()
         self._check(self.L.n4_correct_batch(self.h, arr, len(vols), C.byref(p), reps, None))
         res = []
-        for (shape, _, _, d_out, d_lat, d_lb), r in zip(keep, reps):
-            o = {"normalized": d_out.download(np.float32, shape), "lattice": d_lat.download(np.float32, nc),
-                 "iterations": [int(r.iterations[i]) for i in range(p.number_of_fitting_levels)],
-                 "n_mask_work": int(r.n_mask_work), "elapsed_ms": float(r.elapsed_ms)}
-            if d_lb is not None:
-                o["log_bias"] = d_lb.download(np.float32, shape)
+        for i, ((shape, _, _, d_out, d_lat, d_lb), r) in enumerate(zip(keep, reps)):
+            o = {"iterations": [int(r.iterations[k]) for k in range(p.number_of_fitting_levels)],
+                 "last_cv": [float(r.last_convergence[k]) for k in range(p.number_of_fitting_levels)],
+                 "n_mask_full": int(r.n_mask_full), "n_mask_work": int(r.n_mask_work),
+                 "elapsed_ms": float(r.elapsed_ms)}
+            if sel is None or i in sel:
+                o["normalized"] = d_out.download(np.float32, shape)
+                o["lattice"] = d_lat.download(np.float32, nc)
+                if d_lb is not None:
+                    o["log_bias"] = d_lb.download(np.float32, shape)
             res.append(o)
         return res
 
-    def _batch_arrays(self, vols, p, want_log_bias, shrink):
+    def _batch_arrays(self, vols, p, want_log_bias, shrink, sel=None):
         nc = final_ncpt(p)
         keep, arr = [], (Volume * len(vols))()
         for i, (im, mask) in enumerate(vols):
             im = np.ascontiguousarray(im, dtype=np.float32)
             d_im = self.to_device(im)
-            d_mask = self.to_device(np.ascontiguousarray(mask, dtype=np.float32)) if mask is not None else None
+            if mask is None:
+                d_mask, mk = None, MASK_NONE
+            elif np.asarray(mask).dtype == np.uint8:
+                d_mask, mk = self.to_device(np.ascontiguousarray(mask)), MASK_U8
+            else:
+                d_mask, mk = self.to_device(np.ascontiguousarray(mask, dtype=np.float32)), MASK_F32
             d_out = self.empty(im.nbytes)
             d_lat = self.empty(4 * nc[0] * nc[1] * nc[2])
-            d_lb = self.empty(im.nbytes) if want_log_bias else None
+            d_lb = self.empty(im.nbytes) if want_log_bias and (sel is None or i in sel) else None
             keep.append((im.shape, d_im, d_mask, d_out, d_lat, d_lb))
             v = arr[i]
             v.d_im = d_im.ptr
             v.d_mask = d_mask.ptr if d_mask else None
-            v.mask_kind = MASK_F32 if d_mask else MASK_NONE
+            v.mask_kind = mk
             for d in range(3):
                 v.dims[d] = im.shape[d]
             v.shrink_factor = int(shrink[i]) if shrink is not None else 0

@@ -154,3 +154,17 @@ def batch_shrinks(n: int, seed: int, choices=(2, 3, 4)):
     from `choices` with their own seeded generator."""
     rng = np.random.default_rng(seed)
     return [int(v) for v in rng.choice(np.asarray(choices), size=n)]
+
+
+def generate_batch(specs, workers: int = 16, mask_dtype=np.uint8):
+    """make_batch_volume over specs on a thread pool (numpy releases the GIL
+    in the heavy array work).  Returns [(image float32, mask)] with the mask
+    cast to mask_dtype."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(spec):
+        im, mask, _ = make_batch_volume(spec)
+        return im, mask.astype(mask_dtype)
+
+    with ThreadPoolExecutor(max(1, int(workers))) as ex:
+        return list(ex.map(one, specs))

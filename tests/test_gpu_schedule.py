@@ -88,3 +88,68 @@ def test_autotune_batch(gpu):
             assert np.max(np.abs(a["log_bias"] - b["log_bias"])) < 1e-5
     finally:
         gpu.set_tuning(reset=True)
+
+
+def _group_split(xs, groups):
+    """The library's volume grouping for volumes of dims (x, 16, 16), shrink
+    1 (one row tile per slab, so a volume has x working slabs): largest
+    first onto the least-loaded group.  Returns per group (first volume,
+    volumes, first slab, slabs)."""
+    order = sorted(range(len(xs)), key=lambda v: (-256 * xs[v], v))
+    load, gid = [0] * groups, [0] * len(xs)
+    for v in order:
+        g = load.index(min(load))
+        gid[v] = g
+        load[g] += 256 * xs[v]
+    out, v0, s0 = [], 0, 0
+    for g in range(groups):
+        mem = [v for v in range(len(xs)) if gid[v] == g]
+        n, s = len(mem), sum(xs[v] for v in mem)
+        out.append((v0, n, s0, s))
+        v0, s0 = v0 + n, s0 + s
+    return out
+
+
+def _key_collision(groups=4, nvol=8, seed=11):
+    """Two batches of equal size in which some group g >= 1 starts at the
+    same slab with the same slab and volume counts but at a different first
+    volume: a step graph keyed without the first volume would be replayed
+    with the other batch's volume range."""
+    rng = np.random.default_rng(seed)
+    seen = {}
+    for _ in range(20000):
+        xs = [int(v) for v in rng.integers(6, 31, size=nvol)]
+        for g, (v0, n, s0, s) in enumerate(_group_split(xs, groups)):
+            if g == 0:
+                continue
+            k = (g, n, s0, s, sum(xs))
+            if k in seen and seen[k][1] != v0:
+                return seen[k][0], xs
+            seen.setdefault(k, (xs, v0))
+    return None
+
+
+def test_graph_key_holds_group_first_volume(gpu):
+    """Regression: batches whose group splits differ only in where a group's
+    volumes start must not share a cached step graph."""
+    pair = _key_collision()
+    assert pair is not None
+    batches = []
+    for b, xs in enumerate(pair):
+        vols = []
+        for k, x in enumerate(xs):
+            im, m, _ = PC.small_phantom((x, 16, 16), seed=300 + 10 * b + k)
+            vols.append((im, m))
+        batches.append(vols)
+    kw = dict(levels=2, max_iter=[6, 6], fixed_iterations=True)
+    try:
+        gpu.set_tuning(graph_steps="off", groups=4)
+        refs = [gpu.correct_batch(v, **kw) for v in batches]
+        gpu.set_tuning(graph_steps=4, groups=4)
+        for _ in range(2):
+            for vols, ref in zip(batches, refs):
+                out = gpu.correct_batch(vols, **kw)
+                for a, b in zip(ref, out):
+                    assert np.array_equal(a["log_bias"], b["log_bias"])
+    finally:
+        gpu.set_tuning(reset=True)
