@@ -245,9 +245,26 @@ struct Plan {
     int cx_max[kMaxLevels] = {0};
     int mt[kMaxLevels] = {0}, nt[kMaxLevels] = {0};  // MFMA tiles along y, z per level
     size_t lds_hist = 0, lds_wiener = 0, lds_recon = 0;
+    // persistent path
+    size_t off_work = 0;     // PWork[n_work] in the upload region
+    int n_work = 0;
+    size_t lds_persist = 0;  // dynamic LDS of the persistent kernel (largest volume)
 };
 
 static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
+
+// Row tiles of the step kernels on a working grid W: rows per tile (whole
+// multiples of 16 rows, one MFMA row tile per wave) and tiles per slab.
+// tiled == false (stage calls): whole slabs.
+static void step_tiles(const int W[3], bool tiled, int tile_vox, int* rows, int* tiles_x) {
+    int r = W[1];
+    if (tiled && (size_t)W[1] * W[2] > (size_t)tile_vox) {
+        const int t = (tile_vox / std::max(W[2], 1) + 15) / 16 * 16;
+        r = std::min(W[1], std::max(t, 16));
+    }
+    *rows = r;
+    *tiles_x = (W[1] + r - 1) / r;
+}
 
 
 // Magic multiplier for floor(v / d) (see fdiv in the kernels).
@@ -269,6 +286,8 @@ struct Schedule {
     int tile_vox;     // working voxels per step-kernel tile
     int graph_steps;  // steps per captured graph; 0 = direct launches
     bool mfma_eval;   // MFMA field-increment kernel (MFMA precisions)
+    int path;         // N4_PATH_*
+    int team_per_cu;  // persistent path: workgroups per compute unit (0 = occupancy limit)
 };
 
 static int env_int(const char* name, int dflt) {
@@ -306,6 +325,8 @@ static Schedule resolve_schedule(const n4_handle* h, int nvol) {
     const uint32_t ek = h->tune.eval_kernel ? h->tune.eval_kernel
                                             : (uint32_t)env_int("N4_EVAL_KERNEL", (int)N4_EVAL_FP32);
     s.mfma_eval = ek == N4_EVAL_MFMA;
+    s.path = h->tune.path ? (int)h->tune.path : env_int("N4_PATH", (int)N4_PATH_AUTO);
+    s.team_per_cu = h->tune.team_wgs_per_cu ? (int)h->tune.team_wgs_per_cu : env_int("N4_TEAM_WGS_PER_CU", 0);
     return s;
 }
 
@@ -329,8 +350,13 @@ static int rows_for(int rows, int cols, int target) {
 // the descriptors are arena-relative until `rebase` adds the arena base.
 // tiled: cut working slabs into row tiles for the step kernels (the
 // correction); stage calls keep whole slabs (their partials are per slab).
+// teams (nullable, persistent path): workgroups per volume; the plan then
+// holds the PWork list (consecutive ranks per volume, each rank a
+// contiguous share of the volume's tiles), a barrier slot per volume and
+// histogram copies sized to the team.
 static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, Plan* P,
-                            bool need_tables_full = true, bool tiled = false, int tile_vox = kStepTileVox) {
+                            bool need_tables_full = true, bool tiled = false, int tile_vox = kStepTileVox,
+                            const std::vector<int>* teams = nullptr) {
     const int nv = (int)in.size();
     const int L = (int)p->number_of_fitting_levels;
     P->nvol = nv;
@@ -360,13 +386,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         I.prep_tiles_x = (I.W[1] + I.prep_rows - 1) / I.prep_rows;
         I.recon_rows = rows_for(in[v].F[1], in[v].F[2], kReconTileVox);
         I.recon_tiles_x = (in[v].F[1] + I.recon_rows - 1) / I.recon_rows;
-        I.wtile_rows = I.W[1];
-        if (tiled && (size_t)I.W[1] * I.W[2] > (size_t)tile_vox) {
-            // multiples of 16 rows (one MFMA row tile per wave)
-            const int r = (tile_vox / std::max(I.W[2], 1) + 15) / 16 * 16;
-            I.wtile_rows = std::min(I.W[1], std::max(r, 16));
-        }
-        I.wtiles_x = (I.W[1] + I.wtile_rows - 1) / I.wtile_rows;
+        step_tiles(I.W, tiled, tile_vox, &I.wtile_rows, &I.wtiles_x);
         nws += I.W[0] * I.wtiles_x;
         nps += I.W[0] * I.prep_tiles_x;
         nfs += in[v].F[0] * I.recon_tiles_x;
@@ -384,6 +404,10 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     P->off_pslab = lay.take(sizeof(Slab) * (nps > 0 ? nps : 1));
     P->off_state = lay.take(sizeof(VolState) * nv);
     P->off_tw = lay.take(sizeof(D2) * P->rp.P);
+    P->n_work = 0;
+    if (teams)
+        for (int v = 0; v < nv; ++v) P->n_work += (*teams)[v];
+    P->off_work = lay.take(sizeof(PWork) * std::max(P->n_work, 1));
     // tables
     std::vector<size_t> off_ax(nv * kMaxLevels * 3), off_sup(nv * kMaxLevels * 3), off_fax(nv * 3);
     for (int v = 0; v < nv; ++v)
@@ -399,7 +423,12 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
 
     // per-volume working buffers
     struct Offs {
-        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, numfx, hist, spart, etab, ppart;
+        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, numfx, hist, spart, etab, ppart, team;
+    };
+    // histogram copies: per tile (step kernels) or per team member (teams)
+    auto hcopies = [&](int v) {
+        if (teams) return std::max(1, std::min(kHistCopies, ((*teams)[v] + 7) / 8));
+        return hist_copies(vi[v].W[0] * vi[v].wtiles_x, tiled);
     };
     std::vector<Offs> off(nv);
     const int nb = P->rp.nbins;
@@ -417,10 +446,11 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.lresf = lay.take(sizeof(float) * I.Lmax);
         o.part = lay.take(sizeof(float) * I.W[0] * I.part_stride);
         o.numfx = lay.take(sizeof(uint64_t) * I.Lmax * num_copies(I.W[0] * I.wtiles_x, tiled));
-        o.hist = lay.take(sizeof(uint64_t) * nb * hist_copies(I.W[0] * I.wtiles_x, tiled));
+        o.hist = lay.take(sizeof(uint64_t) * nb * hcopies(v));
         o.spart = lay.take(sizeof(double) * 4 * I.W[0] * I.wtiles_x);
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
         o.etab = lay.take(sizeof(double) * nb);
+        o.team = teams ? lay.take(64) : 0;
     }
     P->total_bytes = lay.take(0);
 
@@ -435,7 +465,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     Slab* ws = (Slab*)(hb + P->off_wslab);
     Slab* fs = (Slab*)(hb + P->off_fslab);
     Slab* ps = (Slab*)(hb + P->off_pslab);
-    int iws = 0, ifs = 0, ips = 0;
+    int iws = 0, ifs = 0, ips = 0, iwk = 0;
     {
         D2* tw = (D2*)(hb + P->off_tw);
         for (int m = 0; m < P->rp.P; ++m) {
@@ -482,7 +512,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.recon_rows = I.recon_rows;
         D.wtile_rows = I.wtile_rows;
         D.wtiles_x = I.wtiles_x;
-        D.hist_copies = hist_copies(I.W[0] * I.wtiles_x, tiled);
+        D.hist_copies = hcopies(v);
+        D.team_ctr = teams ? (uint32_t*)(base + o.team) : nullptr;
         D.mW2 = magic_div((uint32_t)I.W[2]);
         D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
@@ -514,6 +545,14 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             }
         }
         P->desc[v] = D;
+        if (teams) {
+            PWork* wk = (PWork*)(hb + P->off_work);
+            const int nt = (*teams)[v], ntl = I.W[0] * I.wtiles_x;
+            for (int r = 0; r < nt; ++r)
+                wk[iwk++] = PWork{v, r, nt, iws + (int)((int64_t)ntl * r / nt), iws + (int)((int64_t)ntl * (r + 1) / nt),
+                                  iws};
+            P->lds_persist = std::max(P->lds_persist, lds_persist(D, nb, P->rp.P, P->rp.prec));
+        }
         for (int x = 0; x < I.W[0]; ++x) {
             for (int y0 = 0; y0 < I.W[1]; y0 += I.wtile_rows)
                 ws[iws++] = Slab{v, x, y0, std::min(I.wtile_rows, I.W[1] - y0)};
@@ -810,6 +849,74 @@ struct GroupRun {
     bool finished = false;
 };
 
+// Persistent path: team sizes (workgroups per volume) for a batch, or
+// false with the reason when the batch is not eligible.  Every workgroup
+// of the launch must be resident at once, so the teams share the device's
+// resident-workgroup capacity for the kernel's LDS size, in proportion to
+// the volumes' working voxels (at least one workgroup, at most one per
+// tile, per volume).
+static bool plan_teams(const std::vector<VolIn>& in, const n4_params* p, const Schedule& sch, std::vector<int>* teams,
+                       std::string* why) {
+    const int nv = (int)in.size();
+    const int L = (int)p->number_of_fitting_levels;
+#ifdef N4_EMULATE
+    if (p->precision != N4_PREC_FP32) {
+        *why = "the host emulation runs the FP32 fit";
+        return false;
+    }
+#endif
+    if (p->precision != N4_PREC_FP32)
+        for (int d = 1; d < 3; ++d)
+            if (level_ncpt((int)p->ncpt[d], L - 1) > 16) {
+                *why = "the MFMA fit of the persistent path covers up to 16 control points along y and z";
+                return false;
+            }
+    const RunParams rp = make_run_params(p);
+    size_t lds = 0;
+    std::vector<int> tiles(nv);
+    std::vector<double> w(nv);
+    for (int v = 0; v < nv; ++v) {
+        VolDesc D;
+        memset(&D, 0, sizeof D);
+        for (int d = 0; d < 3; ++d) D.W[d] = (in[v].F[d] + in[v].s - 1) / in[v].s;
+        D.nlevels = L;
+        for (int l = 0; l < L; ++l) {
+            for (int d = 0; d < 3; ++d) D.lev[l].C[d] = level_ncpt((int)p->ncpt[d], l);
+            D.lev[l].L = D.lev[l].C[0] * D.lev[l].C[1] * D.lev[l].C[2];
+        }
+        int rows, tx;
+        step_tiles(D.W, true, sch.tile_vox, &rows, &tx);
+        D.wtile_rows = rows;
+        lds = std::max(lds, lds_persist(D, rp.nbins, rp.P, rp.prec));
+        tiles[v] = D.W[0] * tx;
+        w[v] = (double)D.W[0] * D.W[1] * D.W[2];
+    }
+    if (lds > kLdsCap) {
+        *why = "a volume's tables exceed one workgroup's LDS";
+        return false;
+    }
+    int per_cu = 0, cus = 0;
+    if (persist_capacity(lds, rp.prec, &per_cu, &cus) != 0 || per_cu < 1 || cus < 1) {
+        *why = "occupancy query failed";
+        return false;
+    }
+    per_cu = std::min(per_cu, sch.team_per_cu > 0 ? sch.team_per_cu : 4);
+    const int cap = per_cu * cus;
+    if (nv > cap) {
+        *why = "more volumes than resident workgroups";
+        return false;
+    }
+    double tot = 0.0;
+    for (int v = 0; v < nv; ++v) tot += w[v];
+    teams->assign(nv, 1);
+    const int spare = cap - nv;
+    for (int v = 0; v < nv; ++v) {
+        const int extra = tot > 0.0 ? (int)floor((double)spare * w[v] / tot) : 0;
+        (*teams)[v] = std::max(1, std::min(tiles[v], 1 + extra));
+    }
+    return true;
+}
+
 #define GLAUNCH(id, expr)                  \
     do {                                   \
         ProfScope ps_(h, (id), gs);        \
@@ -828,7 +935,15 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     if (in_user.empty()) return N4_OK;
     const int nv = (int)in_user.size();
     const Schedule sch = resolve_schedule(h, nv);
-    const int G = sch.groups;
+    std::vector<int> teams_user;
+    bool teams_path = false;
+    if (sch.path != (int)N4_PATH_STEPS) {
+        std::string why;
+        teams_path = plan_teams(in_user, p, sch, &teams_user, &why);
+        if (!teams_path && sch.path == (int)N4_PATH_TEAMS)
+            return fail(N4_ERR_UNSUPPORTED, "persistent team path not available for this batch: %s", why.c_str());
+    }
+    const int G = teams_path ? 1 : sch.groups;
     // order volumes by group: largest first (working voxels), each onto the
     // group with the least work so far
     std::vector<int> order(nv), gid(nv, 0);
@@ -855,7 +970,11 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                     if (gid[v] == g && prep_class(in_user[v].s) == c) order[k++] = v;
     }
     std::vector<VolIn> in(nv);
-    for (int k = 0; k < nv; ++k) in[k] = in_user[order[k]];
+    std::vector<int> teams(teams_path ? nv : 0);
+    for (int k = 0; k < nv; ++k) {
+        in[k] = in_user[order[k]];
+        if (teams_path) teams[k] = teams_user[order[k]];
+    }
 #ifndef N4_EMULATE
     N4_CHECK_HIP(hipSetDevice(h->device));
 #endif
@@ -877,9 +996,9 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     N4_CHECK_HIP(hipEventRecord(ev.a, (hipStream_t)stream));
 #endif
     Plan P;
-    s = build_plan(h, in, p, &P, true, true, sch.tile_vox);
+    s = build_plan(h, in, p, &P, true, true, sch.tile_vox, teams_path ? &teams : nullptr);
     if (s != N4_OK) return s;
-    s = check_lds(P, true, true);
+    s = check_lds(P, !teams_path, true);
     if (s != N4_OK) return s;
     H2D(h->arena, h->staging.data(), P.upload_bytes, stream);
     VolDesc* dd = dev_desc(h, P);
@@ -964,7 +1083,20 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                         launch_prep(dd, dev_pslab(h, P) + R.pc[c], R.pc[c + 1] - R.pc[c], kPrepClasses[c], rp, gs));
         GLAUNCH(N4_K_LEVEL, launch_level(dd + R.v0, R.v1 - R.v0, 0, rp, gs));
     }
-    int remaining = G;
+    if (teams_path) {
+        // one launch runs every volume's level/iteration loop
+        void* gs = stream;
+        GLAUNCH(N4_K_TEAMS, launch_persist(dd, wsl, (const PWork*)((unsigned char*)h->arena + P.off_work), P.n_work,
+                                           rp, P.lds_persist, gs));
+        GLAUNCH(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, gs));
+        for (int v = 0; v < nv; ++v)
+            if (in[v].out_lattice) {
+                const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
+                D2D(in[v].out_lattice, P.desc[v].latf, sizeof(float) * LD.L, gs);
+            }
+        for (GroupRun& R : gr) R.finished = true;
+    }
+    int remaining = teams_path ? 0 : G;
     uint64_t idle_rounds = 0;
     while (remaining > 0) {
         bool advanced = false;
@@ -1049,6 +1181,9 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     }
     std::vector<VolState> st(nv);
     D2H(st.data(), dev_state(h, P), sizeof(VolState) * nv, stream);
+    std::vector<uint32_t> tctr(teams_path ? 2 * nv : 0);
+    if (teams_path)
+        for (int v = 0; v < nv; ++v) D2H(&tctr[2 * v], P.desc[v].team_ctr, 2 * sizeof(uint32_t), stream);
     float ms = 0.0f;
 #ifndef N4_EMULATE
     N4_CHECK_HIP(hipEventRecord(ev.b, (hipStream_t)stream));
@@ -1057,6 +1192,9 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     N4_CHECK_HIP(hipGetLastError());
 #endif
     prof_collect(h);
+    for (int v = 0; v < (int)tctr.size() / 2; ++v)
+        if (tctr[2 * v + 1])
+            return fail(N4_ERR_INTERNAL, "persistent path: a team barrier of volume %d timed out", order[v]);
     if (reports)
         for (int k = 0; k < nv; ++k) {
             const int v = order[k];
@@ -1244,6 +1382,8 @@ n4_status n4_set_tuning(n4_handle* h, const n4_tuning* t) {
     if (t->graph_steps != N4_GRAPH_OFF && t->graph_steps > (uint32_t)kMaxGraphSteps)
         return fail(N4_ERR_INVALID_ARG, "graph_steps > %d", kMaxGraphSteps);
     if (t->eval_kernel > N4_EVAL_MFMA) return fail(N4_ERR_INVALID_ARG, "unknown eval_kernel %u", t->eval_kernel);
+    if (t->path > N4_PATH_TEAMS) return fail(N4_ERR_INVALID_ARG, "unknown path %u", t->path);
+    if (t->team_wgs_per_cu > 32) return fail(N4_ERR_INVALID_ARG, "team_wgs_per_cu > 32");
     h->tune = *t;
     h->tune.ms = 0.0f;
     return N4_OK;
@@ -1262,8 +1402,8 @@ n4_status n4_autotune_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, 
         if (g > nvol || g > (uint32_t)kMaxGroups) continue;
         for (uint32_t t : {4096u, 8192u, 16384u})
             for (uint32_t k : {N4_GRAPH_OFF, 8u, 16u}) {
-                cand.push_back(n4_tuning{g, t, k, N4_EVAL_FP32, 0.0f});
-                if (p->precision != N4_PREC_FP32) cand.push_back(n4_tuning{g, t, k, N4_EVAL_MFMA, 0.0f});
+                cand.push_back(n4_tuning{g, t, k, N4_EVAL_FP32, N4_PATH_STEPS, 0, 0.0f});
+                if (p->precision != N4_PREC_FP32) cand.push_back(n4_tuning{g, t, k, N4_EVAL_MFMA, N4_PATH_STEPS, 0, 0.0f});
             }
     }
     // best wall time of `reps` timed runs after one untimed run; N4_AUTOTUNE_LOG=1

@@ -121,6 +121,7 @@ struct VolDesc {
     double* ppart;             // prep partials [W0 * prep_tiles_x][4]
     uint64_t mW2;              // magic divisor for the working z extent
     uint64_t mF2;              // magic divisor for the full-resolution z extent
+    uint32_t* team_ctr;        // persistent path: [0] the team's barrier counter, [1] abort flag
 };
 
 struct D2 {
@@ -141,6 +142,18 @@ struct RunParams {
     float low_f;      // largest float <= low: x > low_f  <=>  (double)x > low
     int32_t fixed_iters;
     int32_t prec;     // n4_precision: 0 FP32, 1 BF16 MFMA, 2 split-BF16 MFMA
+};
+
+// Persistent path: one workgroup of a volume's team.  The team's
+// workgroups are consecutive blocks; workgroup `rank` owns the volume's
+// tiles [t0, t1) of the slab list (contiguous in (x, y0) order) for the
+// whole correction.
+struct PWork {
+    int32_t vol;
+    int32_t rank;   // 0 = the team's leader
+    int32_t nteam;  // workgroups in the team
+    int32_t t0, t1;
+    int32_t vt0;    // the volume's first tile in the slab list
 };
 
 // One workgroup's tile: rows [y0, y0+ny) of slab x of volume vol.

@@ -67,6 +67,17 @@ int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp,
 int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds,
                  void* stream);
 
+// Persistent path: the level/iteration loop of every volume in one launch
+// of nwg workgroups (work[b]: team, rank and tiles of block b; a volume's
+// team members are consecutive blocks).  Every workgroup must be resident
+// at once: nwg <= per_cu * cus from persist_capacity for the same LDS size.
+// prec (RunParams::prec): FP32 fit, or the MFMA fit for up to 16 control
+// points along y and z (persist_eligible).
+size_t lds_persist(const VolDesc& v, int nbins, int P, int prec);
+int persist_capacity(size_t lds, int prec, int* per_cu, int* cus);
+int launch_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, int nwg, RunParams rp, size_t lds,
+                   void* stream);
+
 // Flat (1-D) stage helpers used by the stage entry points.  A flat array of
 // n voxels is cut into slabs of kChunkVox voxels; each slab writes
 // spart[x] = 4 doubles (reduced on the host by the stage call).

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 #define __host__
 #define __device__
@@ -74,13 +75,20 @@
 
 #define HD __host__ __device__ inline
 
+// LDS view: every pointer into a workgroup's LDS carries this address space.
+#ifdef N4_DEVICE_PASS
+#define N4_LDS __attribute__((address_space(3)))
+#else
+#define N4_LDS
+#endif
+
 namespace n4 {
 
 // ---------------------------------------------------------------- helpers --
 
 HD D2 cmul(D2 a, D2 b) { return D2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-HD void lds_add_u64(unsigned long long* p, unsigned long long v) {
+HD void lds_add_u64(N4_LDS unsigned long long* p, unsigned long long v) {
 #ifdef N4_DEVICE_PASS
     atomicAdd(p, v);
 #else
@@ -92,7 +100,7 @@ HD void global_add_u64(uint64_t* p, unsigned long long v) {
 #ifdef N4_DEVICE_PASS
     atomicAdd(reinterpret_cast<unsigned long long*>(p), v);
 #else
-    *p += v;
+    __atomic_fetch_add(p, (uint64_t)v, __ATOMIC_SEQ_CST);
 #endif
 }
 
@@ -133,6 +141,30 @@ HD F4 load_quad(const float* p) {
 #endif
 }
 
+// Data shared between the workgroups of a persistent team (partials,
+// histogram and numerator copies, lattice, increments, barrier counter) is
+// written only with agent-scope atomics / stores and read only with
+// agent-scope loads: coherent across the XCDs' L2 caches without cache
+// write-backs or invalidations.
+template <class T>
+HD T ld_agent(const T* p) {
+#ifdef N4_DEVICE_PASS
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+    T v;
+    __atomic_load(p, &v, __ATOMIC_SEQ_CST);
+    return v;
+#endif
+}
+template <class T>
+HD void st_agent(T* p, T v) {
+#ifdef N4_DEVICE_PASS
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+    __atomic_store(p, &v, __ATOMIC_SEQ_CST);
+#endif
+}
+
 // Index of a step-kernel tile within its volume (its slot in spart).
 HD int tile_id(const VolDesc& V, const Slab& sl) {
     return sl.x * V.wtiles_x + (V.wtiles_x > 1 ? sl.y0 / V.wtile_rows : 0);
@@ -144,14 +176,17 @@ HD int round_up(int v, int m) { return (v + m - 1) / m * m; }
 HD uint32_t fdiv(uint32_t v, uint64_t m) { return (uint32_t)(((uint64_t)v * m) >> kDivShift); }
 
 // Bump allocator over the block's dynamic LDS; with base == nullptr it only
-// measures (host-side sizing uses the same code as the device carve).
+// measures (host-side sizing uses the same code as the device carve).  Every
+// carve-out is typed as LDS (N4_LDS), so the compiler rejects passing one
+// where a generic pointer is expected: LDS is only ever read and written
+// with ds_* instructions, never with flat ones.
 struct Carve {
-    unsigned char* base;
+    N4_LDS unsigned char* base;
     size_t off;
     template <class T>
-    HD T* take(size_t n) {
+    HD N4_LDS T* take(size_t n) {
         off = align16(off);
-        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        N4_LDS T* p = base ? reinterpret_cast<N4_LDS T*>(base + off) : nullptr;
         off += n * sizeof(T);
         return p;
     }
@@ -180,7 +215,7 @@ HD double red_apply(int op, double a, double b) {
 // wave order.  The order of operations is fixed, so every block reducing
 // the same data gets the same bits; the host emulation performs the same
 // operations in the same order.
-HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
+HD void tree_reduce4(N4_LDS double* red, int o0, int o1, int o2, int o3) {
     BLOCK_SYNC();
     const int ops[4] = {o0, o1, o2, o3};
     constexpr int kWaves = kBlock / 64;
@@ -227,7 +262,7 @@ HD void tree_reduce4(double* red, int o0, int o1, int o2, int o3) {
 struct alignas(16) D2x {
     double a, b;
 };
-HD void reduce_partials(const double* part, int nslab, double* red) {
+HD void reduce_partials(const double* part, int nslab, N4_LDS double* red) {
     BLOCK_FOR(t)
     double r0 = 0.0, r1 = 0.0, r2 = INFINITY, r3 = -INFINITY;
     for (int x0 = t; x0 < nslab; x0 += 8 * kBlock) {
@@ -356,9 +391,9 @@ HD void prep_commit(const VolDesc& V, int x, int J, int K, int nm, int nb, float
 }
 
 template <int S, int MK>
-HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsigned char* smem) {
+HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS unsigned char* smem) {
     Carve cv{smem, 0};
-    double* red = cv.take<double>(4 * kBlock);
+    N4_LDS double* red = cv.take<double>(4 * kBlock);
     const int W2 = V.W[2];
     const int x = sl.x;
     // Shrink 1 and 2: UN working voxels per thread have all their s^3 image
@@ -445,7 +480,7 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, unsign
 }
 
 template <int S>
-HD void body_prep_s(const VolDesc& V, const Slab sl, const RunParams& rp, unsigned char* smem) {
+HD void body_prep_s(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS unsigned char* smem) {
     if (V.mask_kind == 0)
         body_prep_t<S, 0>(V, sl, rp, smem);
     else if (V.mask_kind == 1)
@@ -454,7 +489,7 @@ HD void body_prep_s(const VolDesc& V, const Slab sl, const RunParams& rp, unsign
         body_prep_t<S, 2>(V, sl, rp, smem);
 }
 
-HD void body_prep(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
+HD void body_prep(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     switch (V.s) {
@@ -476,6 +511,9 @@ HD double refine_w(int odd, int a) {
 // Lattice of `level` (> 0) as the dyadic refinement of level-1's FP64
 // lattice (N4.py:132-165, cudaN4.cu:257-339) with per-axis clip bounds;
 // also its FP32 copy.  Runs inside a thread phase (thread t of kBlock).
+// TEAM (persistent path, run by the team's leader): the lattices are read
+// and written with agent-scope accesses (the team's owners update them).
+template <bool TEAM>
 HD void refine_lattice(const VolDesc& V, int level, int t) {
     const LevelDesc& LD = V.lev[level];
     const LevelDesc& PD = V.lev[level - 1];
@@ -507,12 +545,17 @@ HD void refine_lattice(const VolDesc& V, int level, int t) {
                     const int nz = PD.C[2] - 3;
                     Z = Z < -1 ? -1 : (Z > nz + 1 ? nz + 1 : Z);
                     const double wgt = refine_w(odd[0], a) * refine_w(odd[1], b) * refine_w(odd[2], e);
-                    acc += wgt * src[((size_t)(X + 1) * PD.C[1] + (Y + 1)) * PD.C[2] + (Z + 1)];
+                    const double* sp = src + ((size_t)(X + 1) * PD.C[1] + (Y + 1)) * PD.C[2] + (Z + 1);
+                    acc += wgt * (TEAM ? ld_agent(sp) : *sp);
                 }
             }
         }
-        dst[l] = acc;
-        V.latf[l] = (float)acc;
+        if (TEAM) {
+            st_agent(dst + l, acc);
+        } else {
+            dst[l] = acc;
+            V.latf[l] = (float)acc;
+        }
     }
 }
 
@@ -522,17 +565,17 @@ HD void refine_lattice(const VolDesc& V, int level, int t) {
 //    range, clears lattice and accumulators and puts the volume in state
 //    TRANS(0).
 //  level > 0: refines the level-1 lattice (lattice refinement stage call).
-HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsigned char* smem) {
+HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const VolDesc& V = vols[bid];
     VolState* st = V.st;
     if (level > 0) {
         BLOCK_FOR(t)
-        refine_lattice(V, level, t);
+        refine_lattice<false>(V, level, t);
         BLOCK_END
         return;
     }
     Carve cv{smem, 0};
-    double* red = cv.take<double>(4 * kBlock);
+    N4_LDS double* red = cv.take<double>(4 * kBlock);
     reduce_partials(V.ppart, V.W[0] * V.prep_tiles_x, red);
     const LevelDesc& LD = V.lev[0];
     int Lmax = 0;
@@ -552,6 +595,7 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
     }
     for (int l = t; l < Lmax * V.num_copies; l += kBlock) V.numfx[l] = 0;
     for (int i = t; i < rp.nbins * V.hist_copies; i += kBlock) V.hist[i] = 0;
+    if (t < 2 && V.team_ctr) V.team_ctr[t] = 0;
     if (t == 0) {
         st->n_mask_full = (uint64_t)red[0];
         st->n_mask_work = (uint64_t)red[kBlock];
@@ -574,6 +618,78 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
 }
 
 // ------------------------------------------------------------- histogram --
+// Fractional histogram of one tile into kHistLanes lane-strided LDS copies
+// of the bins (odd stride hs: the copies of one bin fall in different
+// banks), N4.py:19-29: bin index c = (x - min) / slope in FP64 exactly as
+// N4.py computes it, contributions as 2^-32 fixed point (integer adds:
+// order-independent).  Aligned quads of voxels cover the tile: one 16-byte
+// logU load and one 4-byte mask load per quad, four quads in flight per
+// thread.  dbg_idx (nullable): per-voxel bin index, -1 where unmasked.
+HD void hist_zero(N4_LDS unsigned long long* h, int hs) {
+    BLOCK_FOR(t)
+    for (int i = t; i < kHistLanes * hs; i += kBlock) h[i] = 0ull;
+    BLOCK_END
+}
+
+HD void hist_tile(const VolDesc& V, const Slab sl, double mn, double slope, int nb, N4_LDS unsigned long long* h,
+                  int hs, int32_t* dbg_idx) {
+    const int YZ = sl.ny * V.W[2];
+    const size_t base = ((size_t)sl.x * V.W[1] + sl.y0) * V.W[2];
+    const size_t vend = std::min(base + (size_t)YZ, (size_t)V.nvox_w);
+    const size_t q0 = base & ~(size_t)3;
+    const int nq = (int)((vend - q0 + 3) >> 2);
+    BLOCK_FOR(t)
+    N4_LDS unsigned long long* hl = h + (size_t)(t & (kHistLanes - 1)) * hs;
+    for (int i0 = 0; i0 < nq; i0 += 4 * kBlock) {
+        F4 xs[4];
+        uint32_t ms[4];
+        UNROLL for (int u = 0; u < 4; ++u) {
+            const int qi = i0 + u * kBlock + t;
+            const size_t w = q0 + 4 * (size_t)qi;
+            const bool ok = qi < nq;
+            ms[u] = ok ? *reinterpret_cast<const N4_GLOBAL uint32_t*>(gp(V.wmask) + w) : 0u;
+            xs[u] = ok ? load_quad(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
+        }
+        UNROLL for (int u = 0; u < 4; ++u) {
+            const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
+            UNROLL for (int j = 0; j < 4; ++j) {
+                const size_t w = wq + j;
+                if (w < base || w >= vend) continue;
+                if (!((ms[u] >> (8 * j)) & 0xffu)) {
+                    if (dbg_idx) dbg_idx[w] = -1;
+                    continue;
+                }
+                const double c = ((double)xs[u].v[j] - mn) / slope;
+                const double fl = floor(c);
+                const int idx = (fl >= 0.0 && fl < (double)nb) ? (int)fl : -1;
+                if (dbg_idx) dbg_idx[w] = idx;
+                if (idx < 0) continue;
+                const double off = c - fl;
+                if (off == 0.0) {
+                    lds_add_u64(&hl[idx], 1ull << 32);
+                } else if (idx < nb - 1) {
+                    const unsigned long long q = (unsigned long long)rint(off * 4294967296.0);
+                    lds_add_u64(&hl[idx], (1ull << 32) - q);
+                    lds_add_u64(&hl[idx + 1], q);
+                }
+            }
+        }
+    }
+    BLOCK_END
+}
+
+// Merges the LDS copies and adds them to one global copy of the bins.
+HD void hist_flush(const N4_LDS unsigned long long* h, int hs, int nb, uint64_t* dst) {
+    BLOCK_FOR(t)
+    for (int i = t; i < nb; i += kBlock) {
+        unsigned long long s = 0;
+        for (int c = 0; c < kHistLanes; ++c) s += h[(size_t)c * hs + i];
+        if (s) global_add_u64(&dst[i], s);
+    }
+    BLOCK_END
+}
+
+
 // Prologue (every block of the volume computes identical bits): reduce the
 // eval partials of the previous step -> logU min/max (N4.py:14) and the
 // convergence measurement of the previous iteration, then the step's kind
@@ -592,17 +708,17 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, unsign
 // u64 bins, then in the volume's global u64 bins (integer adds:
 // order-independent, so bitwise deterministic).
 HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp, int32_t* dbg_idx, int bid,
-                  unsigned char* smem) {
+                  N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     VolState* st = V.st;
     const int nb = rp.nbins;
     Carve cv{smem, 0};
-    double* red = cv.take<double>(4 * kBlock);
+    N4_LDS double* red = cv.take<double>(4 * kBlock);
     // kHistLanes lane-strided copies of the bins (odd stride: the copies of
     // one bin fall in different LDS banks), merged before the global flush
     const int hs = nb | 1;
-    unsigned long long* h = cv.take<unsigned long long>((size_t)kHistLanes * hs);
+    N4_LDS unsigned long long* h = cv.take<unsigned long long>((size_t)kHistLanes * hs);
     const int mode = step < 0 ? kModeIter : st->mode;
     const int lvl = step < 0 ? 0 : st->lvl, it = step < 0 ? 0 : st->it;
     if (mode == kModeFinished) {
@@ -659,65 +775,12 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
         BLOCK_END
     }
     if (kind != kStepIter || degenerate) return;
-    const int YZ = sl.ny * V.W[2];
-    const size_t base = ((size_t)sl.x * V.W[1] + sl.y0) * V.W[2];
-    BLOCK_FOR(t)
-    for (int i = t; i < kHistLanes * hs; i += kBlock) h[i] = 0ull;
-    BLOCK_END
+    hist_zero(h, hs);
     BLOCK_SYNC();
-    // aligned quads of voxels covering [base, vend): one 16-byte logU load
-    // and one 4-byte mask load per quad, four quads in flight per thread
-    const size_t vend = std::min(base + (size_t)YZ, (size_t)V.nvox_w);
-    const size_t q0 = base & ~(size_t)3;
-    const int nq = (int)((vend - q0 + 3) >> 2);
-    BLOCK_FOR(t)
-    unsigned long long* hl = h + (size_t)(t & (kHistLanes - 1)) * hs;
-    for (int i0 = 0; i0 < nq; i0 += 4 * kBlock) {
-        F4 xs[4];
-        uint32_t ms[4];
-        UNROLL for (int u = 0; u < 4; ++u) {
-            const int qi = i0 + u * kBlock + t;
-            const size_t w = q0 + 4 * (size_t)qi;
-            const bool ok = qi < nq;
-            ms[u] = ok ? *reinterpret_cast<const N4_GLOBAL uint32_t*>(gp(V.wmask) + w) : 0u;
-            xs[u] = ok ? load_quad(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
-        }
-        UNROLL for (int u = 0; u < 4; ++u) {
-            const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
-            UNROLL for (int j = 0; j < 4; ++j) {
-                const size_t w = wq + j;
-                if (w < base || w >= vend) continue;
-                if (!((ms[u] >> (8 * j)) & 0xffu)) {
-                    if (dbg_idx) dbg_idx[w] = -1;
-                    continue;
-                }
-                const double c = ((double)xs[u].v[j] - mn) / slope;
-                const double fl = floor(c);
-                const int idx = (fl >= 0.0 && fl < (double)nb) ? (int)fl : -1;
-                if (dbg_idx) dbg_idx[w] = idx;
-                if (idx < 0) continue;
-                const double off = c - fl;
-                if (off == 0.0) {
-                    lds_add_u64(&hl[idx], 1ull << 32);
-                } else if (idx < nb - 1) {
-                    const unsigned long long q = (unsigned long long)rint(off * 4294967296.0);
-                    lds_add_u64(&hl[idx], (1ull << 32) - q);
-                    lds_add_u64(&hl[idx + 1], q);
-                }
-            }
-        }
-    }
-    BLOCK_END
+    hist_tile(V, sl, mn, slope, nb, h, hs, dbg_idx);
     BLOCK_SYNC();
     // tiles spread their flush over the copies: fewer atomics per address
-    uint64_t* dst = V.hist + (size_t)(tile_id(V, sl) % V.hist_copies) * nb;
-    BLOCK_FOR(t)
-    for (int i = t; i < nb; i += kBlock) {
-        unsigned long long s = 0;
-        for (int c = 0; c < kHistLanes; ++c) s += h[(size_t)c * hs + i];
-        if (s) global_add_u64(&dst[i], s);
-    }
-    BLOCK_END
+    hist_flush(h, hs, nb, V.hist + (size_t)(tile_id(V, sl) % V.hist_copies) * nb);
 }
 
 // ---------------------------------------------------------------- Wiener --
@@ -769,7 +832,7 @@ HD void dft_small(D2 (&v)[R], double sg) {
 // and writes b[(j - k) R + k + r Ns].
 // The same pass run by one wavefront (lane l of 64).
 template <int R>
-HD void wave_fft_pass(const D2* a, D2* b, const D2* tw, int P, int Ns, bool inverse) {
+HD void wave_fft_pass(const N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, int Ns, bool inverse) {
     const int nbf = P / R;
     const int tstep = P / (Ns * R);
     const double sg = inverse ? 1.0 : -1.0;
@@ -796,7 +859,7 @@ HD void wave_fft_pass(const D2* a, D2* b, const D2* tw, int P, int Ns, bool inve
 // or 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P, in LDS.  Run by
 // wavefront 0 alone.  Returns the buffer holding the result (the other one
 // is free).
-HD D2* wave_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
+HD N4_LDS D2* wave_fft(N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, bool inverse) {
     for (int Ns = 1; Ns < P;) {
         const int rem = P / Ns;
         const int R = rem >= 8 ? 8 : rem;
@@ -807,7 +870,7 @@ HD D2* wave_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
         else
             wave_fft_pass<2>(a, b, tw, P, Ns, inverse);
         WAVE_SYNC();
-        D2* tmp = a;
+        N4_LDS D2* tmp = a;
         a = b;
         b = tmp;
         Ns *= R;
@@ -819,13 +882,18 @@ HD D2* wave_fft(D2* a, D2* b, const D2* tw, int P, bool inverse) {
 // N4.py:31-101 (cudaN4.cu:733-820 did this on the host with FFTW).  Two real
 // transforms share one complex FFT (V in the real part, F in the
 // imaginary).  Reads and clears the volume's histogram.
-HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, unsigned char* smem) {
+// TEAM (persistent path): the copies are read with agent-scope loads and
+// left for the team's leader to clear, and the table goes to Ef (FP32, LDS:
+// the remap's copy) instead of V.etab.
+template <bool TEAM>
+HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, N4_LDS unsigned char* smem,
+                     N4_LDS float* Ef = nullptr) {
     const int P = rp.P, half = P / 2, off = rp.offset, nb = rp.nbins;
     Carve cv{smem, 0};
-    D2* z = cv.take<D2>(P);
-    D2* y = cv.take<D2>(P);
-    D2* ff = cv.take<D2>(P);
-    D2* tw = cv.take<D2>(P);
+    N4_LDS D2* z = cv.take<D2>(P);
+    N4_LDS D2* y = cv.take<D2>(P);
+    N4_LDS D2* ff = cv.take<D2>(P);
+    N4_LDS D2* tw = cv.take<D2>(P);
     const double scaledFWHM = rp.fwhm / slope;
     const double expF = 4.0 * log(2.0) / (scaledFWHM * scaledFWHM);
     const double scale = 2.0 * sqrt(log(2.0) / M_PI) / scaledFWHM;
@@ -839,11 +907,17 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
             uint64_t* hp = V.hist + (j - off);
             const int nc = V.hist_copies;
             uint64_t c[kMaxCopies];
-            UNROLL for (int k = 0; k < kMaxCopies; ++k) c[k] = k < nc ? hp[(size_t)k * nb] : 0;
+            UNROLL for (int k = 0; k < kMaxCopies; ++k) {
+                if (TEAM)
+                    c[k] = k < nc ? ld_agent(hp + (size_t)k * nb) : 0;
+                else
+                    c[k] = k < nc ? hp[(size_t)k * nb] : 0;
+            }
             uint64_t s = 0;
             UNROLL for (int k = 0; k < kMaxCopies; ++k) s += c[k];
-            UNROLL for (int k = 0; k < kMaxCopies; ++k)
-                if (k < nc) hp[(size_t)k * nb] = 0;
+            if (!TEAM)
+                UNROLL for (int k = 0; k < kMaxCopies; ++k)
+                    if (k < nc) hp[(size_t)k * nb] = 0;
             Vj = (double)s * 0x1p-32;
         }
         z[j] = D2{Vj, Fj};
@@ -853,8 +927,8 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
     BLOCK_SYNC();
     // the sequential chain of transforms and pointwise steps: wavefront 0
     WAVE0_BEGIN
-    D2* Z = wave_fft(z, y, tw, P, false);
-    D2* U = (Z == z) ? y : z;
+    N4_LDS D2* Z = wave_fft(z, y, tw, P, false);
+    N4_LDS D2* U = (Z == z) ? y : z;
     // Split Vf and Ff, Gf = conj(Ff)/(conj(Ff)Ff + noise), Uf = Vf*Re(Gf)
     WAVE_FOR(l)
     for (int k = l; k < P; k += 64) {
@@ -868,8 +942,8 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
     }
     WAVE_END
     WAVE_SYNC();
-    D2* Ur = wave_fft(U, Z, tw, P, true);
-    D2* Wb = (Ur == U) ? Z : U;
+    N4_LDS D2* Ur = wave_fft(U, Z, tw, P, true);
+    N4_LDS D2* Wb = (Ur == U) ? Z : U;
     const double invP = 1.0 / (double)P;
     WAVE_FOR(l)
     for (int j = l; j < P; j += 64) {
@@ -879,18 +953,22 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
     }
     WAVE_END
     WAVE_SYNC();
-    D2* Wf = wave_fft(Wb, Ur, tw, P, false);
-    D2* Wo = (Wf == Wb) ? Ur : Wb;
+    N4_LDS D2* Wf = wave_fft(Wb, Ur, tw, P, false);
+    N4_LDS D2* Wo = (Wf == Wb) ? Ur : Wb;
     WAVE_FOR(l)
     for (int k = l; k < P; k += 64) Wf[k] = cmul(Wf[k], ff[k]);
     WAVE_END
     WAVE_SYNC();
-    D2* R = wave_fft(Wf, Wo, tw, P, true);
+    N4_LDS D2* R = wave_fft(Wf, Wo, tw, P, true);
     WAVE_FOR(l)
     for (int i = l; i < nb; i += 64) {
         const double nu = R[off + i].x * invP;
         const double de = R[off + i].y * invP;
-        V.etab[i] = (de != 0.0) ? nu / de : 0.0;
+        const double e = (de != 0.0) ? nu / de : 0.0;
+        if (TEAM)
+            Ef[i] = (float)e;
+        else
+            V.etab[i] = e;
     }
     WAVE_END
     WAVE0_END
@@ -901,14 +979,14 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
 //   ITER   Wiener step (unless degenerate); next: ITER(l, it+1)
 //   TRANS  refine the lattice into level l (l > 0); next: ITER(l, 0)
 //   FINAL  FP32 copy of the final lattice; next: finished
-HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* smem) {
+HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const VolDesc& V = vols[bid];
     VolState* st = V.st;
     // the state fields of the step, in one round of loads
     const int kind = st->s_kind, degenerate = st->degenerate, s_it = st->s_it;
     const double bmin = st->bin_min, slope = st->slope;
     if (kind == kStepIter) {
-        if (!degenerate) wiener_table(V, bmin, slope, rp, smem);
+        if (!degenerate) wiener_table<false>(V, bmin, slope, rp, smem);
         BLOCK_FOR(t)
         if (t == 0) {
             st->mode = kModeIter;
@@ -918,7 +996,7 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, unsigned char* s
     } else if (kind == kStepTrans) {
         const int l = st->s_lvl;
         BLOCK_FOR(t)
-        if (l > 0) refine_lattice(V, l, t);
+        if (l > 0) refine_lattice<false>(V, l, t);
         if (t == 0) {
             st->mode = kModeIter;
             st->lvl = l;
@@ -1009,15 +1087,15 @@ HD void body_update(const VolDesc* vols, RunParams rp, int chunk, int vol) {
 // Modes: NUM (g = residual, P = w^3/S), DEN (g = mask, Q = w^2; once per
 // level), FIELD (g = the logU buffer itself; stage API).
 struct FitLds {
-    float* E;
-    float* zwc;  // [Cz][zmax]  factor of control point cz at z = lo(cz) + i
-    int* zs;
-    int* ys;
-    float* g;
-    float* T1;
-    float* T2;
-    float* ywc;  // [Cy][ymax]  factor of control point cy at row y = lo(cy) + i
-    AxisEntry* ex;  // the slab's x factors (copied in the set-up round)
+    N4_LDS float* E;
+    N4_LDS float* zwc;  // [Cz][zmax]  factor of control point cz at z = lo(cz) + i
+    N4_LDS int* zs;
+    N4_LDS int* ys;
+    N4_LDS float* g;
+    N4_LDS float* T1;
+    N4_LDS float* T2;
+    N4_LDS float* ywc;  // [Cy][ymax]  factor of control point cy at row y = lo(cy) + i
+    N4_LDS AxisEntry* ex;  // the slab's x factors (copied in the set-up round)
 };
 
 // Longest support (in grid points) of one cubic B-spline basis function over
@@ -1052,12 +1130,12 @@ HD bool fit_step_mode(const VolState* st, int* mode, int* level, int* it) {
 // the rounding logU itself carries; the histogram bin index, which must be
 // exact, stays FP64 in the histogram kernel.
 struct Remap {
-    const float* E;
+    const N4_LDS float* E;
     float bmin, islope;
     int nb;
 };
 
-HD Remap make_remap(const float* E, double bmin, double slope, int nb) {
+HD Remap make_remap(const N4_LDS float* E, double bmin, double slope, int nb) {
     return Remap{E, (float)bmin, (float)(1.0 / slope), nb};
 }
 
@@ -1077,7 +1155,7 @@ HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const Remap& 
     return xv - sh;
 }
 
-HD size_t carve_fit(unsigned char* base, int nbins, int Y, int Z, int R, int Cy, int Cz, FitLds* o) {
+HD size_t carve_fit(N4_LDS unsigned char* base, int nbins, int Y, int Z, int R, int Cy, int Cz, FitLds* o) {
     Carve c{base, 0};
     FitLds L;
     L.E = c.take<float>(nbins);
@@ -1093,35 +1171,15 @@ HD size_t carve_fit(unsigned char* base, int nbins, int Y, int Z, int R, int Cy,
     return align16(c.off);
 }
 
-HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int mode, RunParams rp, int bid,
-                 unsigned char* smem) {
-    const Slab sl = slabs[bid];
-    const VolDesc& V = vols[sl.vol];
-    const VolState* st = V.st;
-    if (!fit_step_mode(st, &mode, &level, &it)) return;
-    const LevelDesc& LD = V.lev[level];
-    const int Y = V.W[1], Z = V.W[2];
+// Thread t's share of a fit's factor tables: per control point, its
+// factors (w^2 for the denominator, else w^3/sum w^2) over its support as
+// one contiguous run (the contraction loops then read two independent LDS
+// streams), and the support ranges.
+HD void fit_tables_t(const LevelDesc& LD, const FitLds& S, int Y, int Z, bool den, int t) {
     const int Cy = LD.C[1], Cz = LD.C[2];
-    const int R = rows_per_chunk(Y, Z);
-    const int nb = rp.nbins;
-    FitLds S;
-    carve_fit(smem, nb, Y, Z, R, Cy, Cz, &S);
     const int zmax = support_len(Z, Cz), ymax = support_len(Y, Cy);
-    const bool num = (mode == kFitNum);
-    const bool den = (mode == kFitDen || mode == kFitDenScatter);
-    const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
-    const int gmode = den ? (int)kFitDen : mode;
-    const bool degenerate = num && st->degenerate;
-    const double bmin = st->bin_min, slope = st->slope;
-    const Remap rm = make_remap(S.E, bmin, slope, nb);
     const AxisEntry* axz = LD.ax[2];
     const AxisEntry* axy = LD.ax[1];
-    const int x = sl.x;
-    // per control point, its factors over its support as one contiguous run
-    // (the contraction loops then read two independent LDS streams)
-    BLOCK_FOR(t)
-    if (num)
-        for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
     for (int e = t; e < Cz * zmax; e += kBlock) {
         const int c = e / zmax, i = e - c * zmax;
         const int z = LD.sup[2][2 * c] + i;
@@ -1146,12 +1204,20 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
     }
     for (int i = t; i < 2 * Cz; i += kBlock) S.zs[i] = LD.sup[2][i];
     for (int i = t; i < 2 * Cy; i += kBlock) S.ys[i] = LD.sup[1][i];
-    for (int i = t; i < Cy * Cz; i += kBlock) S.T2[i] = 0.0f;
-    // the slab's x factors for the final scatter, in this round of loads
-    if (t < (int)(sizeof(AxisEntry) / 4))
-        reinterpret_cast<int32_t*>(S.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][x])[t];
-    BLOCK_END
-    BLOCK_SYNC();
+}
+
+// The (y, z) contraction of one tile into S.T2 (zeroed by the caller):
+// per chunk of R rows, g = fit_input of each voxel (aligned quads, 16-byte
+// logU loads), T1[r,cz] = sum_z g Pz over the control point's contiguous
+// support run, T2[cy,cz] += sum_y Py T1.  Tables (zwc, ywc, zs, ys) and, for
+// NUM, the E table must be in LDS.
+HD void fit_chunks(const VolDesc& V, const LevelDesc& LD, const Slab sl, const FitLds& S, int gmode, bool den,
+                   bool degenerate, const Remap& rm) {
+    const int Y = V.W[1], Z = V.W[2];
+    const int Cy = LD.C[1], Cz = LD.C[2];
+    const int R = rows_per_chunk(Y, Z);
+    const int zmax = support_len(Z, Cz), ymax = support_len(Y, Cy);
+    const int x = sl.x;
     const int yend = sl.y0 + sl.ny;
     for (int y0 = sl.y0; y0 < yend; y0 += R) {
         const int ny = (yend - y0 < R) ? yend - y0 : R;
@@ -1186,8 +1252,8 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
             const int lo = S.zs[2 * c], n = S.zs[2 * c + 1] - lo;
-            const float* gr = S.g + (size_t)r * Z + lo;
-            const float* wc = S.zwc + (size_t)c * zmax;
+            const N4_LDS float* gr = S.g + (size_t)r * Z + lo;
+            const N4_LDS float* wc = S.zwc + (size_t)c * zmax;
             float acc = 0.0f;
             for (int i = 0; i < n; ++i) acc += gr[i] * wc[i];
             S.T1[pr] = acc;
@@ -1201,23 +1267,64 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
             int lo = ylo, hi = S.ys[2 * cy + 1];
             lo = lo > y0 ? lo : y0;
             hi = hi < y0 + ny ? hi : y0 + ny;
-            const float* wc = S.ywc + (size_t)cy * ymax - ylo;
+            const N4_LDS float* wc = S.ywc + (size_t)cy * ymax;  // wc[i]: row ylo + i
             float acc = S.T2[pr];
-            for (int yy = lo; yy < hi; ++yy) acc += wc[yy] * S.T1[(yy - y0) * Cz + c];
+            for (int yy = lo; yy < hi; ++yy) acc += wc[yy - ylo] * S.T1[(yy - y0) * Cz + c];
             S.T2[pr] = acc;
         }
         BLOCK_END
         BLOCK_SYNC();
     }
+}
+
+// Adds a tile's T2, weighted by the slab's x factors, to the numerator
+// (scatter: fixed-point atomics into copy tile % num_copies) or writes it
+// as the slab's partial (stage calls).
+HD void fit_scatter(const VolDesc& V, const LevelDesc& LD, const Slab sl, const FitLds& S, bool scatter, bool den) {
+    const int CyCz = LD.C[1] * LD.C[2];
     BLOCK_FOR(t)
     const AxisEntry ex = *S.ex;
-    for (int pr = t; pr < Cy * Cz; pr += kBlock) {
+    for (int pr = t; pr < CyCz; pr += kBlock) {
         if (scatter)
             scatter_numerator(V, LD, ex, tile_id(V, sl) % V.num_copies, den, pr, S.T2[pr]);
         else
-            V.part[(size_t)x * V.part_stride + pr] = S.T2[pr];
+            V.part[(size_t)sl.x * V.part_stride + pr] = S.T2[pr];
     }
     BLOCK_END
+}
+
+HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int mode, RunParams rp, int bid,
+                 N4_LDS unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    const VolState* st = V.st;
+    if (!fit_step_mode(st, &mode, &level, &it)) return;
+    const LevelDesc& LD = V.lev[level];
+    const int Y = V.W[1], Z = V.W[2];
+    const int Cy = LD.C[1], Cz = LD.C[2];
+    const int nb = rp.nbins;
+    FitLds S;
+    carve_fit(smem, nb, Y, Z, rows_per_chunk(Y, Z), Cy, Cz, &S);
+    const bool num = (mode == kFitNum);
+    const bool den = (mode == kFitDen || mode == kFitDenScatter);
+    const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
+    const int gmode = den ? (int)kFitDen : mode;
+    const bool degenerate = num && st->degenerate;
+    const double bmin = st->bin_min, slope = st->slope;
+    const Remap rm = make_remap(S.E, bmin, slope, nb);
+    const int x = sl.x;
+    BLOCK_FOR(t)
+    if (num)
+        for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
+    fit_tables_t(LD, S, Y, Z, den, t);
+    for (int i = t; i < Cy * Cz; i += kBlock) S.T2[i] = 0.0f;
+    // the slab's x factors for the final scatter, in this round of loads
+    if (t < (int)(sizeof(AxisEntry) / 4))
+        reinterpret_cast<N4_LDS int32_t*>(S.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][x])[t];
+    BLOCK_END
+    BLOCK_SYNC();
+    fit_chunks(V, LD, sl, S, gmode, den, degenerate, rm);
+    fit_scatter(V, LD, sl, S, scatter, den);
 }
 
 // --------------------------------------------------------------- lattice --
@@ -1230,13 +1337,13 @@ HD void body_fit(const VolDesc* vols, const Slab* slabs, int level, int it, int 
 // Each column p is summed by G = kBlock/CyCz threads over interleaved slabs,
 // then combined in a fixed order.
 HD void body_lattice(const VolDesc* vols, int level, int mode, RunParams rp, int cx, int vol,
-                     unsigned char* smem) {
+                     N4_LDS unsigned char* smem) {
     const VolDesc& V = vols[vol];
     const LevelDesc& LD = V.lev[level];
     if (cx >= LD.C[0]) return;
     if (mode == kFitNum && V.st->done) return;
     Carve cv{smem, 0};
-    double* red = cv.take<double>(kBlock);
+    N4_LDS double* red = cv.take<double>(kBlock);
     const int CyCz = LD.C[1] * LD.C[2];
     const int lo = LD.sup[0][2 * cx], hi = LD.sup[0][2 * cx + 1];
     const AxisEntry* axx = LD.ax[0];
@@ -1290,16 +1397,16 @@ HD void body_lattice(const VolDesc* vols, int level, int mode, RunParams rp, int
 //   logU -= d  (= imlog - new field);  u = exp(old-new) - 1 = expm1(-d)
 // Per slab: spart[x] = (sum u, sum u^2 in FP64, min, max of logU).
 struct EvalLds {
-    float* Ar;
-    float* Br;
-    float* zw;
-    int* zb;
-    float* yw;  // [tile rows][4] y factors of the tile's rows
-    int* yb;
-    double* red;
+    N4_LDS float* Ar;
+    N4_LDS float* Br;
+    N4_LDS float* zw;
+    N4_LDS int* zb;
+    N4_LDS float* yw;  // [tile rows][4] y factors of the tile's rows
+    N4_LDS int* yb;
+    N4_LDS double* red;
 };
 
-HD size_t carve_eval(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, EvalLds* o) {
+HD size_t carve_eval(N4_LDS unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, EvalLds* o) {
     Carve c{base, 0};
     EvalLds L;
     L.Ar = c.take<float>((size_t)Cy * Cz);
@@ -1313,7 +1420,78 @@ HD size_t carve_eval(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, 
     return align16(c.off);
 }
 
-HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
+// Field increment of one tile from its x-contracted increment planes
+// S.Ar (and the tile-relative y tables S.yw, S.yb): per chunk of rows
+// Br[r,cz] = sum_b wy Ar[by+b,cz], then per voxel d = sum_c wz Br (z tables
+// S.zw, S.zb), logU -= d, FP64 sums of u = expm1(-d) and FP32 min/max of
+// the new logU accumulated in S.red (initialised by the caller: 0, 0, +inf,
+// -inf), then reduced in a fixed order: red[0] sum u, red[kBlock] sum u^2,
+// red[2 kBlock] min, red[3 kBlock] max.
+HD void eval_chunks(const VolDesc& V, const LevelDesc& LD, const Slab sl, const EvalLds& S) {
+    const int Y = V.W[1], Z = V.W[2];
+    const int Cz = LD.C[2];
+    const int R = rows_per_eval_chunk(Y, Z);
+    const int x = sl.x;
+    const int yend = sl.y0 + sl.ny;
+    for (int y0 = sl.y0; y0 < yend; y0 += R) {
+        const int ny = (yend - y0 < R) ? yend - y0 : R;
+        const size_t base = ((size_t)x * Y + y0) * Z;
+        BLOCK_FOR(t)
+        for (int pr = t; pr < ny * Cz; pr += kBlock) {
+            const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
+            const int tr = y0 - sl.y0 + r;  // tile row
+            const N4_LDS float* yw = S.yw + 4 * tr;
+            const int b0 = S.yb[tr];
+            float br = 0.0f;
+            for (int k = 0; k < 4; ++k) br += yw[k] * S.Ar[(b0 + k) * Cz + c];
+            S.Br[pr] = br;
+        }
+        BLOCK_END
+        BLOCK_SYNC();
+        BLOCK_FOR(t)
+        double s1 = S.red[t], s2 = S.red[kBlock + t];
+        // min/max of FP32 values are exact in FP32
+        float mn = (float)S.red[2 * kBlock + t], mx = (float)S.red[3 * kBlock + t];
+        // consecutive lanes on consecutive voxels (conflict-free LDS
+        // reads of the z tables), kEvalLoads mask and logU loads in flight
+        // per thread, all independent (logU is read whatever the mask)
+        constexpr int kEvalLoads = 8;
+        for (int v0 = 0; v0 < ny * Z; v0 += kEvalLoads * kBlock) {
+            uint8_t ms[kEvalLoads];
+            float lu[kEvalLoads];
+            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
+                const int v = v0 + u * kBlock + t;
+                const bool ok = v < ny * Z;
+                ms[u] = ok ? gp(V.wmask)[base + v] : 0;
+                lu[u] = ok ? gp(V.logU)[base + v] : 0.0f;
+            }
+            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
+                if (!ms[u]) continue;
+                const int v = v0 + u * kBlock + t;
+                const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
+                const N4_LDS float* wz = S.zw + 4 * z;
+                const N4_LDS float* brr = S.Br + r * Cz + S.zb[z];
+                const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
+                const float lg = lu[u] - d;
+                gp(V.logU)[base + v] = lg;
+                const double uu = (double)expm1f(-d);
+                s1 += uu;
+                s2 += uu * uu;
+                mn = fminf(mn, lg);
+                mx = fmaxf(mx, lg);
+            }
+        }
+        S.red[t] = s1;
+        S.red[kBlock + t] = s2;
+        S.red[2 * kBlock + t] = mn;
+        S.red[3 * kBlock + t] = mx;
+        BLOCK_END
+        BLOCK_SYNC();
+    }
+    tree_reduce4(S.red, 0, 0, 1, 2);
+}
+
+HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     if (V.st->s_kind != kStepIter) return;
@@ -1351,63 +1529,7 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
     S.red[3 * kBlock + t] = -INFINITY;
     BLOCK_END
     BLOCK_SYNC();
-    const int yend = sl.y0 + sl.ny;
-    for (int y0 = sl.y0; y0 < yend; y0 += R) {
-        const int ny = (yend - y0 < R) ? yend - y0 : R;
-        const size_t base = ((size_t)x * Y + y0) * Z;
-        BLOCK_FOR(t)
-        for (int pr = t; pr < ny * Cz; pr += kBlock) {
-            const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
-            const int tr = y0 - sl.y0 + r;  // tile row
-            const float* yw = S.yw + 4 * tr;
-            const int b0 = S.yb[tr];
-            float br = 0.0f;
-            for (int k = 0; k < 4; ++k) br += yw[k] * S.Ar[(b0 + k) * Cz + c];
-            S.Br[pr] = br;
-        }
-        BLOCK_END
-        BLOCK_SYNC();
-        BLOCK_FOR(t)
-        double s1 = S.red[t], s2 = S.red[kBlock + t];
-        // min/max of FP32 values are exact in FP32
-        float mn = (float)S.red[2 * kBlock + t], mx = (float)S.red[3 * kBlock + t];
-        // consecutive lanes on consecutive voxels (conflict-free LDS
-        // reads of the z tables), kEvalLoads mask and logU loads in flight
-        // per thread, all independent (logU is read whatever the mask)
-        constexpr int kEvalLoads = 8;
-        for (int v0 = 0; v0 < ny * Z; v0 += kEvalLoads * kBlock) {
-            uint8_t ms[kEvalLoads];
-            float lu[kEvalLoads];
-            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
-                const int v = v0 + u * kBlock + t;
-                const bool ok = v < ny * Z;
-                ms[u] = ok ? gp(V.wmask)[base + v] : 0;
-                lu[u] = ok ? gp(V.logU)[base + v] : 0.0f;
-            }
-            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
-                if (!ms[u]) continue;
-                const int v = v0 + u * kBlock + t;
-                const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
-                const float* wz = S.zw + 4 * z;
-                const float* brr = S.Br + r * Cz + S.zb[z];
-                const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
-                const float lg = lu[u] - d;
-                gp(V.logU)[base + v] = lg;
-                const double uu = (double)expm1f(-d);
-                s1 += uu;
-                s2 += uu * uu;
-                mn = fminf(mn, lg);
-                mx = fmaxf(mx, lg);
-            }
-        }
-        S.red[t] = s1;
-        S.red[kBlock + t] = s2;
-        S.red[2 * kBlock + t] = mn;
-        S.red[3 * kBlock + t] = mx;
-        BLOCK_END
-        BLOCK_SYNC();
-    }
-    tree_reduce4(S.red, 0, 0, 1, 2);
+    eval_chunks(V, LD, sl, S);
     BLOCK_FOR(t)
     if (t == 0) {
         double* sp = V.spart + (size_t)tile_id(V, sl) * 4;
@@ -1424,12 +1546,12 @@ HD void body_eval(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid,
 // cudaN4.cu:1700-1729): separable evaluation, then normalised image
 // (thresholded input / exp(field)), optional exp(field) and field.
 struct ReconLds {
-    float* A;
-    float* B;
-    float* zw;
-    int* zb;
-    float* yw;  // [tile rows][4] y factors of the tile's rows
-    int* yb;
+    N4_LDS float* A;
+    N4_LDS float* B;
+    N4_LDS float* zw;
+    N4_LDS int* zb;
+    N4_LDS float* yw;  // [tile rows][4] y factors of the tile's rows
+    N4_LDS int* yb;
 };
 
 // Rows per y-contracted chunk of a reconstruction tile: the whole tile when
@@ -1442,7 +1564,7 @@ HD int recon_chunk_rows(int tile_rows, int Cz) {
     return r < tile_rows ? r : tile_rows;
 }
 
-HD size_t carve_recon(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, ReconLds* o) {
+HD size_t carve_recon(N4_LDS unsigned char* base, int Z, int R, int Yt, int Cy, int Cz, ReconLds* o) {
     Carve c{base, 0};
     ReconLds L;
     L.A = c.take<float>((size_t)Cy * Cz);
@@ -1456,7 +1578,7 @@ HD size_t carve_recon(unsigned char* base, int Z, int R, int Yt, int Cy, int Cz,
 }
 
 HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams rp, int bid,
-                   unsigned char* smem) {
+                   N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     const LevelDesc& LD = V.lev[level];
@@ -1496,7 +1618,7 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
         for (int pr = t; pr < ny * Cz; pr += kBlock) {
             const int r = (int)fdiv((uint32_t)pr, LD.mCz), c = pr - r * Cz;
             const int tr = y0 - sl.y0 + r;  // tile row
-            const float* yw = S.yw + 4 * tr;
+            const N4_LDS float* yw = S.yw + 4 * tr;
             const int b0 = S.yb[tr];
             float b = 0.0f;
             for (int k = 0; k < 4; ++k) b += yw[k] * S.A[(b0 + k) * Cz + c];
@@ -1522,8 +1644,8 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
                 const int v = v0 + u * kBlock + t;
                 if (v >= ny * Z) continue;
                 const int r = (int)fdiv((uint32_t)v, V.mF2), z = v - r * Z;
-                const float* wz = S.zw + 4 * z;
-                const float* br = S.B + r * Cz + S.zb[z];
+                const N4_LDS float* wz = S.zw + 4 * z;
+                const N4_LDS float* br = S.B + r * Cz + S.zb[z];
                 const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
                 if (o_lb) o_lb[base + v] = f;
                 if (o_n || o_b) {
@@ -1548,12 +1670,12 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
 //   minmax (count, count, min, max); sum (sum, count, min, max);
 //   cv (count, sum u, sum u^2, -) with u = exp(a-b) - 1; remap writes out.
 HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a, const float* b,
-                  float* out, RunParams rp, int bid, unsigned char* smem) {
+                  float* out, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     Carve cv{smem, 0};
-    double* red = cv.take<double>(4 * kBlock);
-    double* E = cv.take<double>(rp.nbins);
+    N4_LDS double* red = cv.take<double>(4 * kBlock);
+    N4_LDS double* E = cv.take<double>(rp.nbins);
     const int64_t n = V.nvox_w;
     const int64_t v0 = (int64_t)sl.x * kChunkVox;
     const int nb = rp.nbins;
@@ -1639,6 +1761,491 @@ HD void body_flat(const VolDesc* vols, const Slab* slabs, int op, const float* a
 
 namespace n4 {
 
+// ============================================================ persistent ==
+// The whole level/iteration loop of every volume in one launch (between the
+// prep + level-0 set-up launches and the reconstruction launch), in place
+// of the host-paced chain of five launches per step.  Each volume has a
+// team of consecutive workgroups; workgroup `rank` owns the volume's tiles
+// [t0, t1) for the whole correction and runs them through the same tile
+// code as the step kernels, so the results are bitwise those of the step
+// path.  Every workgroup of a team runs the volume's state machine itself:
+// the decisions (step kind, convergence, early exit, N4.py:296-322) come
+// from the same reduction of the same partials in the same order, so they
+// agree without being broadcast.  Per iteration step (reference loop
+// cudaN4.cu:1517-1657):
+//   B1  eval partials of the previous step are in: reduce (min, max, CV) ->
+//       step kind
+//       histogram of the own tiles -> one fixed-point copy of the bins
+//   B2  histogram complete: each workgroup sums the copies and runs the
+//       Wiener step itself (E table in its LDS); fit numerator of the own
+//       tiles -> fixed-point copies
+//   B3  numerator complete: each workgroup updates the control points it
+//       owns (a 1/nteam share: lres = num/(den+eps), lattice += lres, lresf;
+//       copies cleared); the leader clears the histogram copies
+//   B4  increments complete: each workgroup loads the increment planes its
+//       tiles touch, evaluates its tiles (logU -= d) and publishes the tiles'
+//       partials
+// A level transition refines the lattice (leader), scatters the level's
+// denominator (own tiles) and, after a barrier, moves it to den (owners).
+constexpr uint32_t kSpinLimit = 1u << 23;  // polls of a team barrier before it gives up (seconds)
+
+HD void spin_pause() {
+#if defined(N4_DEVICE_PASS)
+    __builtin_amdgcn_s_sleep(2);
+#elif defined(N4_EMULATE)
+    std::this_thread::yield();
+#endif
+}
+
+// Team barrier number gen + 1.  Every wave first waits for its own memory
+// operations to complete (every cross-workgroup write is an agent-scope
+// atomic or store, so completion is visibility); the workgroup then
+// synchronises and thread 0 arrives on the team's counter and polls it
+// until all nteam arrivals are in.  Bounded: after kSpinLimit polls, or
+// once a teammate has given up, it raises the team's abort flag and
+// returns false (the workgroup then leaves; the host reports the error).
+HD bool team_barrier(const VolDesc& V, int nteam, uint32_t& gen, N4_LDS int* bc) {
+#ifdef N4_DEVICE_PASS
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#endif
+    BLOCK_SYNC();
+    BLOCK_FOR(t)
+    if (t == 0) {
+        int ok = 1;
+        if (nteam > 1) {
+#ifdef N4_DEVICE_PASS
+            __hip_atomic_fetch_add(V.team_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+            __atomic_fetch_add(V.team_ctr, 1u, __ATOMIC_SEQ_CST);
+#endif
+            const uint32_t target = (gen + 1u) * (uint32_t)nteam;
+            for (uint32_t spins = 1; ld_agent(V.team_ctr) < target; ++spins) {
+                spin_pause();
+                if ((spins & 63u) == 0 && (spins > kSpinLimit || ld_agent(V.team_ctr + 1) != 0)) {
+                    ok = 0;
+                    break;
+                }
+            }
+            if (!ok) st_agent(V.team_ctr + 1, 1u);
+        } else if (ld_agent(V.team_ctr + 1) != 0) {
+            ok = 0;
+        }
+        bc[0] = ok;
+    }
+    BLOCK_END
+    BLOCK_SYNC();
+    ++gen;
+    return bc[0] != 0;
+}
+
+// Reduction of the tiles' partial quadruples, as reduce_partials (same
+// fold order, same results) but with agent-scope loads.
+HD void reduce_partials_team(const double* part, int nslab, N4_LDS double* red) {
+    BLOCK_FOR(t)
+    double r0 = 0.0, r1 = 0.0, r2 = INFINITY, r3 = -INFINITY;
+    for (int x0 = t; x0 < nslab; x0 += 8 * kBlock) {
+        double v[8][4];
+        UNROLL for (int u = 0; u < 8; ++u) {
+            const int xx = x0 + u * kBlock;
+            const double* sp = part + (size_t)(xx < nslab ? xx : 0) * 4;
+            UNROLL for (int k = 0; k < 4; ++k) v[u][k] = ld_agent(sp + k);
+        }
+        UNROLL for (int u = 0; u < 8; ++u) {
+            if (x0 + u * kBlock >= nslab) break;
+            r0 += v[u][0];
+            r1 += v[u][1];
+            r2 = fmin(r2, v[u][2]);
+            r3 = fmax(r3, v[u][3]);
+        }
+    }
+    red[t] = r0;
+    red[kBlock + t] = r1;
+    red[2 * kBlock + t] = r2;
+    red[3 * kBlock + t] = r3;
+    BLOCK_END
+    tree_reduce4(red, 0, 0, 1, 2);
+}
+
+// The control points a team member owns at a level: [o0, o1).
+HD void owned_range(int L, int rank, int nteam, int* o0, int* o1) {
+    *o0 = (int)((int64_t)L * rank / nteam);
+    *o1 = (int)((int64_t)L * (rank + 1) / nteam);
+}
+
+// Owners' share of a step's lattice update, as body_update: num = the sum
+// of the numerator copies (cleared for the next scatter); TRANS: den = num;
+// ITER: lres = num/(den+eps) (N4.py:236), lattice += lres (N4.py:317),
+// lresf = lres.
+HD void owner_update(const VolDesc& V, int level, bool trans, double eps, int o0, int o1) {
+    BLOCK_FOR(t)
+    for (int l = o0 + t; l < o1; l += kBlock) {
+        uint64_t* p = V.numfx + l;
+        const int nc = V.num_copies;
+        uint64_t c[kMaxCopies];
+        UNROLL for (int k = 0; k < kMaxCopies; ++k) c[k] = k < nc ? ld_agent(p + (size_t)k * V.lmax) : 0;
+        uint64_t sum = 0;
+        UNROLL for (int k = 0; k < kMaxCopies; ++k) sum += c[k];
+        UNROLL for (int k = 0; k < kMaxCopies; ++k)
+            if (k < nc) st_agent(p + (size_t)k * V.lmax, (uint64_t)0);
+        const double num = (double)(long long)sum / kNumScale;
+        if (trans) {
+            st_agent(V.den + l, num);
+        } else {
+            const double lres = num / (ld_agent(V.den + l) + eps);
+            double* lat = V.lat[level & 1] + l;
+            st_agent(lat, ld_agent(lat) + lres);
+            st_agent(V.lresf + l, (float)lres);
+        }
+    }
+    BLOCK_END
+}
+
+struct PersistLds {
+    N4_LDS double* red;  // [4 kBlock]
+    N4_LDS int* bc;      // barrier broadcast
+    N4_LDS float* lres;  // (union) increment planes [pa, pa + np) x Cy*Cz of the current level
+    FitLds F;            // E, zwc, zs, ys, ywc, ex persist; g, T1, T2 in the union
+    EvalLds Ev;          // zw, zb, yw, yb (whole y extent) persist; Ar, Br in the union
+    N4_LDS unsigned long long* h;  // (union) histogram copies
+    N4_LDS unsigned char* u;       // the union: histogram | Wiener | fit chunk | increments + eval chunk
+#ifndef N4_EMULATE
+    FitMfmaLds M;  // MFMA fit (prec > 0): E, Pz, ex persist; Py (tile rows), red, Gh, Gl in the union
+#endif
+};
+
+// Largest level value of C[d] * support_len(n, C[d]) (the fit's per-level
+// factor runs).
+HD size_t max_support_run(const VolDesc& V, int d, int n) {
+    size_t m = 0;
+    for (int l = 0; l < V.nlevels; ++l) {
+        const size_t r = (size_t)V.lev[l].C[d] * support_len(n, V.lev[l].C[d]);
+        m = r > m ? r : m;
+    }
+    return m;
+}
+
+// prec > 0: the fit runs on MFMA (one 16-wide tile of control points along
+// y and z) instead of the FP32 factor runs.
+HD size_t carve_persist(N4_LDS unsigned char* base, const VolDesc& V, int nbins, int P, int prec, PersistLds* o) {
+    const int Y = V.W[1], Z = V.W[2];
+    const LevelDesc& LF = V.lev[V.nlevels - 1];
+    const int Cy = LF.C[1], Cz = LF.C[2];
+    Carve c{base, 0};
+    PersistLds L;
+    L.red = c.take<double>((size_t)4 * kBlock);
+    L.bc = c.take<int>(4);
+    L.F.E = c.take<float>(nbins);
+    L.F.ex = c.take<AxisEntry>(1);
+    if (prec == 0) {
+        L.F.zwc = c.take<float>(max_support_run(V, 2, Z));
+        L.F.zs = c.take<int>((size_t)2 * Cz);
+        L.F.ys = c.take<int>((size_t)2 * Cy);
+        L.F.ywc = c.take<float>(max_support_run(V, 1, Y));
+    }
+#ifndef N4_EMULATE
+    if (prec > 0) {
+        L.M.E = L.F.E;
+        L.M.ex = L.F.ex;
+        const int zs = round_up(Z, 32) + 8;
+        L.M.Pz.stride = zs;
+        L.M.Pz.hi = c.take<unsigned short>((size_t)16 * zs);
+        L.M.Pz.lo = c.take<unsigned short>((size_t)16 * zs);
+    }
+#endif
+    L.Ev.zw = c.take<float>((size_t)4 * Z);
+    L.Ev.zb = c.take<int>(Z);
+    L.Ev.yw = c.take<float>((size_t)4 * Y);
+    L.Ev.yb = c.take<int>(Y);
+    L.Ev.red = L.red;
+    c.off = align16(c.off);
+    const size_t u0 = c.off;
+    L.u = base ? base + u0 : nullptr;
+    size_t um = 0;
+    {  // histogram
+        Carve h{L.u, 0};
+        L.h = h.take<unsigned long long>((size_t)kHistLanes * (nbins | 1));
+        um = h.off > um ? h.off : um;
+    }
+    {  // Wiener (carved again by wiener_table from the union base)
+        const size_t w = align16(4 * (size_t)P * sizeof(D2));
+        um = w > um ? w : um;
+    }
+    if (prec == 0) {  // fit chunk
+        Carve f{L.u, 0};
+        const int R = rows_per_chunk(Y, Z);
+        L.F.g = f.take<float>((size_t)R * Z);
+        L.F.T1 = f.take<float>((size_t)R * Cz);
+        L.F.T2 = f.take<float>((size_t)Cy * Cz);
+        um = f.off > um ? f.off : um;
+    }
+#ifndef N4_EMULATE
+    if (prec > 0) {  // MFMA fit of a tile: Py over its rows, wave combine, g staging
+        Carve f{L.u, 0};
+        const int ys = round_up(V.wtile_rows, 16) + 4;
+        L.M.Py.stride = ys;
+        L.M.Py.hi = f.take<unsigned short>((size_t)16 * ys);
+        L.M.Py.lo = f.take<unsigned short>((size_t)16 * ys);
+        L.M.red = f.take<float>(256);
+        L.M.Gh = f.take<unsigned short>((size_t)4 * 16 * kGStride);
+        L.M.Gl = prec == 2 ? f.take<unsigned short>((size_t)4 * 16 * kGStride) : nullptr;
+        um = f.off > um ? f.off : um;
+    }
+#endif
+    {  // eval: the increment planes the tiles touch (at most all of them), then the chunk
+        Carve e{L.u, 0};
+        L.lres = e.take<float>((size_t)LF.L);
+        L.Ev.Ar = e.take<float>((size_t)Cy * Cz);
+        L.Ev.Br = e.take<float>((size_t)rows_per_eval_chunk(Y, Z) * Cz);
+        um = e.off > um ? e.off : um;
+    }
+    if (o) *o = L;
+    return align16(u0 + um);
+}
+
+// The level's tables that persist across its iterations: the fit's factor
+// runs (w^2 for the denominator, else w^3/sum w^2) and the evaluation's z
+// and y weights.
+template <int PREC>
+HD void persist_tables(const VolDesc& V, const LevelDesc& LD, const PersistLds& S, bool den) {
+    const int Y = V.W[1], Z = V.W[2];
+    BLOCK_FOR(t)
+    if constexpr (PREC == 0) {
+        fit_tables_t(LD, S.F, Y, Z, den, t);
+    } else {
+#ifndef N4_EMULATE
+        fit_mfma_tables_t<1, 1>(LD, S.M, Z, 0, 0, den ? 2 : 1, true, false, t);
+#endif
+    }
+    if (!den) {
+        for (int z = t; z < Z; z += kBlock) {
+            const AxisEntry& e = LD.ax[2][z];
+            for (int k = 0; k < 4; ++k) S.Ev.zw[4 * z + k] = e.w[k];
+            S.Ev.zb[z] = e.b;
+        }
+        for (int y = t; y < Y; y += kBlock) {
+            const AxisEntry& e = LD.ax[1][y];
+            for (int k = 0; k < 4; ++k) S.Ev.yw[4 * y + k] = e.w[k];
+            S.Ev.yb[y] = e.b;
+        }
+    }
+    BLOCK_END
+    BLOCK_SYNC();
+}
+
+// Fit of the own tiles (NUM: residual with the LDS E table; DEN: mask with
+// the w^2 factors), scattered into the numerator copies as the step
+// kernels do.
+template <int PREC>
+HD void persist_fit(const VolDesc& V, const LevelDesc& LD, const Slab* my, int nmy, const PersistLds& S, bool den,
+                    bool degenerate, const Remap& rm) {
+    const int CyCz = LD.C[1] * LD.C[2];
+    const int gmode = den ? (int)kFitDen : (int)kFitNum;
+#ifndef N4_EMULATE
+    if constexpr (PREC > 0) {
+        const int t = (int)threadIdx.x, wv = t >> 6;
+        for (int i = 0; i < nmy; ++i) {
+            const Slab sl = my[i];
+            uint8_t ms[16];
+            float xs[16];
+            fit_mfma_load(V, sl, 16 * wv, 0, den, ms, xs);
+            fit_mfma_tables_t<1, 1>(LD, S.M, V.W[2], sl.y0, sl.ny, den ? 2 : 1, false, true, t);
+            if (t < (int)(sizeof(AxisEntry) / 4))
+                reinterpret_cast<N4_LDS int32_t*>(S.M.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][sl.x])[t];
+            __syncthreads();
+            fit_mfma_tile<PREC, 1, 1>(V, LD, sl, S.M, gmode, den, true, degenerate, rm, ms, xs);
+            __syncthreads();
+        }
+        return;
+    }
+#endif
+    for (int i = 0; i < nmy; ++i) {
+        const Slab sl = my[i];
+        BLOCK_FOR(t)
+        for (int k = t; k < CyCz; k += kBlock) S.F.T2[k] = 0.0f;
+        if (t < (int)(sizeof(AxisEntry) / 4))
+            reinterpret_cast<N4_LDS int32_t*>(S.F.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][sl.x])[t];
+        BLOCK_END
+        BLOCK_SYNC();
+        fit_chunks(V, LD, sl, S.F, gmode, den, degenerate, rm);
+        fit_scatter(V, LD, sl, S.F, true, den);
+        BLOCK_SYNC();
+    }
+}
+
+template <int PREC>
+HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, RunParams rp, int bid,
+                     N4_LDS unsigned char* smem) {
+    const PWork W = work[bid];
+    const VolDesc& V = vols[W.vol];
+    VolState* st = V.st;
+    const bool leader = W.rank == 0;
+    const int nb = rp.nbins;
+    const int ntiles = V.W[0] * V.wtiles_x;
+    const Slab* my = slabs + W.t0;
+    const int nmy = W.t1 - W.t0;
+    PersistLds S;
+    carve_persist(smem, V, nb, rp.P, PREC, &S);
+    const double n = (double)st->n_mask_work;  // written by the level-0 set-up launch
+    uint32_t gen = 0;
+    int mode = kModeTrans, lvl = 0, it = 0;
+    int pa = 0;  // first increment plane the own tiles touch (current level)
+    bool need_b1 = false;
+    for (;;) {
+        if (need_b1 && !team_barrier(V, W.nteam, gen, S.bc)) return;  // B1
+        reduce_partials_team(V.spart, ntiles, S.red);
+        const double mn = S.red[2 * kBlock], mx = S.red[3 * kBlock];
+        double cvv = 0.0;
+        int kind = kStepTrans, s_lvl = lvl;
+        bool level_end = false;
+        if (mode == kModeIter) {
+            if (it > 0) cvv = cv_from_sums(S.red[0], S.red[kBlock], n);
+            const bool stop = it > 0 && !rp.fixed_iters && !(cvv > rp.thr);
+            level_end = stop || it >= V.max_iter[lvl];
+            if (!level_end) {
+                kind = kStepIter;
+            } else if (lvl + 1 < V.nlevels) {
+                kind = kStepTrans;
+                s_lvl = lvl + 1;
+            } else {
+                kind = kStepFinal;
+            }
+        }
+        const double slope = (mx - mn) / (double)(nb - 1);
+        const bool degenerate = !(n > 0.0) || !(slope > 0.0);
+        BLOCK_SYNC();  // S.red is read above before any later phase rewrites it
+        if (leader) {
+            BLOCK_FOR(t)
+            if (t == 0) {
+                if (level_end) {
+                    st->iters[lvl] = it;
+                    if (it > 0) {
+                        st->lastcv[lvl] = cvv;
+                        st->cv = cvv;
+                    }
+                }
+                st->s_kind = kind;
+                st->s_lvl = s_lvl;
+                st->s_it = it;
+                st->bin_min = mn;
+                st->bin_max = mx;
+                st->slope = slope;
+                st->degenerate = degenerate ? 1 : 0;
+                st->passes += 1;
+            }
+            BLOCK_END
+        }
+        if (kind == kStepFinal) {
+            if (leader) {
+                const LevelDesc& FD = V.lev[V.nlevels - 1];
+                const double* fin = V.lat[(V.nlevels - 1) & 1];
+                BLOCK_FOR(t)
+                for (int l = t; l < FD.L; l += kBlock) V.latf[l] = (float)ld_agent(fin + l);
+                if (t == 0) {
+                    st->mode = kModeFinished;
+                    st->done = 1;
+                }
+                BLOCK_END
+            }
+            return;
+        }
+        if (kind == kStepTrans) {
+            const int l = s_lvl;
+            const LevelDesc& LD = V.lev[l];
+            if (l > 0 && leader) {
+                BLOCK_FOR(t)
+                refine_lattice<true>(V, l, t);
+                BLOCK_END
+            }
+            persist_tables<PREC>(V, LD, S, true);
+            persist_fit<PREC>(V, LD, my, nmy, S, true, false, Remap{S.F.E, 0.0f, 0.0f, nb});
+            if (!team_barrier(V, W.nteam, gen, S.bc)) return;
+            int o0, o1;
+            owned_range(LD.L, W.rank, W.nteam, &o0, &o1);
+            owner_update(V, l, true, rp.eps, o0, o1);
+            persist_tables<PREC>(V, LD, S, false);
+            pa = nmy > 0 ? LD.ax[0][my[0].x].b : 0;
+            mode = kModeIter;
+            lvl = l;
+            it = 0;
+            need_b1 = false;  // the partials are unchanged
+            continue;
+        }
+        // ---- ITER(lvl, it)
+        const LevelDesc& LD = V.lev[lvl];
+        const int CyCz = LD.C[1] * LD.C[2];
+        const int hs = nb | 1;
+        if (!degenerate) {
+            hist_zero(S.h, hs);
+            BLOCK_SYNC();
+            for (int i = 0; i < nmy; ++i) hist_tile(V, my[i], mn, slope, nb, S.h, hs, nullptr);
+            BLOCK_SYNC();
+            hist_flush(S.h, hs, nb, V.hist + (size_t)(W.rank % V.hist_copies) * nb);
+        }
+        if (!team_barrier(V, W.nteam, gen, S.bc)) return;  // B2
+        if (!degenerate) {
+            wiener_table<true>(V, mn, slope, rp, S.u, S.F.E);
+            BLOCK_SYNC();
+        }
+        persist_fit<PREC>(V, LD, my, nmy, S, false, degenerate, make_remap(S.F.E, mn, slope, nb));
+        if (!team_barrier(V, W.nteam, gen, S.bc)) return;  // B3
+        if (leader) {
+            BLOCK_FOR(t)
+            for (int i = t; i < V.hist_copies * nb; i += kBlock) st_agent(V.hist + i, (uint64_t)0);
+            BLOCK_END
+        }
+        {
+            int o0, o1;
+            owned_range(LD.L, W.rank, W.nteam, &o0, &o1);
+            owner_update(V, lvl, false, rp.eps, o0, o1);
+        }
+        if (!team_barrier(V, W.nteam, gen, S.bc)) return;  // B4
+        if (nmy > 0) {
+            const int pb = LD.ax[0][my[nmy - 1].x].b + 3;
+            BLOCK_FOR(t)
+            for (int i = t; i < (pb - pa + 1) * CyCz; i += kBlock)
+                S.lres[i] = ld_agent(V.lresf + (size_t)pa * CyCz + i);
+            BLOCK_END
+            BLOCK_SYNC();
+        }
+        for (int i = 0; i < nmy; ++i) {
+            const Slab sl = my[i];
+            const AxisEntry ex = LD.ax[0][sl.x];
+            const N4_LDS float* LR = S.lres + (size_t)(ex.b - pa) * CyCz;
+            EvalLds E = S.Ev;
+            E.yw = S.Ev.yw + 4 * sl.y0;  // tile-relative y tables
+            E.yb = S.Ev.yb + sl.y0;
+            BLOCK_FOR(t)
+            for (int pr = t; pr < CyCz; pr += kBlock) {
+                float ar = 0.0f;
+                for (int k = 0; k < 4; ++k) ar += ex.w[k] * LR[k * CyCz + pr];
+                E.Ar[pr] = ar;
+            }
+            E.red[t] = 0.0;
+            E.red[kBlock + t] = 0.0;
+            E.red[2 * kBlock + t] = INFINITY;
+            E.red[3 * kBlock + t] = -INFINITY;
+            BLOCK_END
+            BLOCK_SYNC();
+            eval_chunks(V, LD, sl, E);
+            BLOCK_FOR(t)
+            if (t == 0) {
+                double* sp = V.spart + (size_t)tile_id(V, sl) * 4;
+                st_agent(sp, E.red[0]);
+                st_agent(sp + 1, E.red[kBlock]);
+                st_agent(sp + 2, E.red[2 * kBlock]);
+                st_agent(sp + 3, E.red[3 * kBlock]);
+            }
+            BLOCK_END
+            BLOCK_SYNC();
+        }
+        it += 1;
+        need_b1 = true;
+    }
+}
+
+
 // ---------------------------------------------------------------- sizing --
 
 // MFMA tiles of 16 control points along one axis (0: not an MFMA launch).
@@ -1683,10 +2290,14 @@ size_t lds_recon(const VolDesc& v) {
 
 int mfma_tiles(int c) { return cp_tiles(c); }
 
+size_t lds_persist(const VolDesc& v, int nbins, int P, int prec) {
+    return carve_persist(nullptr, v, nbins, P, prec, nullptr);
+}
+
 // --------------------------------------------------------------- launches --
 #ifndef N4_EMULATE
 
-#define SMEM() reinterpret_cast<unsigned char*>(smem_)
+#define SMEM() ((N4_LDS unsigned char*)(smem_))
 
 // One instance per shrink factor 1..4 (S = 0: any factor), so each keeps
 // the register count of its own case.
@@ -1789,6 +2400,42 @@ static bool mfma_ok(const RunParams& rp, int mt, int nt) {
 }
 
 static int last_launch_status() { return (int)hipGetLastError(); }
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_persist(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs,
+                                                    const PWork* __restrict__ work, RunParams rp) {
+    HIP_DYNAMIC_SHARED(double, smem_);
+    body_persist<PREC>(vols, slabs, work, rp, blockIdx.x, SMEM());
+}
+
+int persist_capacity(size_t lds, int prec, int* per_cu, int* cus) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) {
+        if (prec == 1)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_persist<1>, kBlock, lds);
+        else if (prec == 2)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_persist<2>, kBlock, lds);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_persist<0>, kBlock, lds);
+    }
+    return (int)e;
+}
+
+int launch_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, int nwg, RunParams rp, size_t lds,
+                   void* stream) {
+    if (nwg <= 0) return 0;
+    const dim3 g(nwg), b(kBlock);
+    const hipStream_t s = (hipStream_t)stream;
+    if (rp.prec == 1)
+        hipLaunchKernelGGL(k_persist<1>, g, b, lds, s, vols, slabs, work, rp);
+    else if (rp.prec == 2)
+        hipLaunchKernelGGL(k_persist<2>, g, b, lds, s, vols, slabs, work, rp);
+    else
+        hipLaunchKernelGGL(k_persist<0>, g, b, lds, s, vols, slabs, work, rp);
+    return last_launch_status();
+}
 
 int launch_prep(const VolDesc* vols, const Slab* slabs, int nslab, int shrink, RunParams rp, void* stream) {
     if (nslab <= 0) return 0;
@@ -1942,6 +2589,27 @@ int launch_eval(const VolDesc* vols, const Slab* slabs, int nslab, RunParams rp,
 int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, RunParams rp, size_t lds, void*) {
     unsigned char* b = emu_lds(lds);
     for (int i = 0; i < nslab; ++i) body_recon(vols, slabs, level, rp, i, b);
+    return 0;
+}
+
+// Workgroups of the persistent kernel wait for each other at team
+// barriers, so each runs on a host thread of its own.
+int persist_capacity(size_t, int, int* per_cu, int* cus) {
+    *per_cu = 4;
+    *cus = 16;
+    return 0;
+}
+
+int launch_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, int nwg, RunParams rp, size_t lds,
+                   void*) {
+    std::vector<std::thread> th;
+    th.reserve(nwg);
+    for (int b = 0; b < nwg; ++b)
+        th.emplace_back([=] {
+            std::vector<unsigned char> buf(lds + 64, 0);
+            body_persist<0>(vols, slabs, work, rp, b, buf.data());
+        });
+    for (std::thread& x : th) x.join();
     return 0;
 }
 

@@ -87,8 +87,8 @@ __device__ inline void to_op4(f32x4 v, i16x4& hi, i16x4& lo) {
 
 // BF16 factor table in LDS: rows of `stride` halves, hi then (PREC 2) lo.
 struct Tab {
-    unsigned short* hi;
-    unsigned short* lo;
+    N4_LDS unsigned short* hi;
+    N4_LDS unsigned short* lo;
     int stride;
     __device__ void put(int r, int c, float v) const {
         unsigned short h, l;
@@ -96,11 +96,11 @@ struct Tab {
         hi[r * stride + c] = h;
         if (lo) lo[r * stride + c] = l;
     }
-    __device__ i16x4 get4(const unsigned short* t, int r, int c) const {
-        return *reinterpret_cast<const i16x4*>(t + r * stride + c);
+    __device__ i16x4 get4(const N4_LDS unsigned short* t, int r, int c) const {
+        return *reinterpret_cast<const N4_LDS i16x4*>(t + r * stride + c);
     }
-    __device__ u16x8 get8(const unsigned short* t, int r, int c) const {
-        return *reinterpret_cast<const u16x8*>(t + r * stride + c);
+    __device__ u16x8 get8(const N4_LDS unsigned short* t, int r, int c) const {
+        return *reinterpret_cast<const N4_LDS u16x8*>(t + r * stride + c);
     }
 };
 
@@ -132,13 +132,13 @@ __device__ inline void fill_axis(const Tab& T, bool cp_rows, int pos, int ncp_pa
 
 // ------------------------------------------------------------------- fit --
 struct FitMfmaLds {
-    float* E;
+    N4_LDS float* E;
     Tab Pz;   // [NT*16][Zp + 8]  Pz^T: row = control point, column = z
     Tab Py;   // [MT*16][Yp + 4]  Py^T: row = control point, column = y
-    float* red;
-    unsigned short* Gh;  // [4 waves][16][kGStride]  staged g, BF16 hi
-    unsigned short* Gl;  // same, lo (PREC 2)
-    AxisEntry* ex;       // the slab's x factors (copied in the set-up round)
+    N4_LDS float* red;
+    N4_LDS unsigned short* Gh;  // [4 waves][16][kGStride]  staged g, BF16 hi
+    N4_LDS unsigned short* Gl;  // same, lo (PREC 2)
+    N4_LDS AxisEntry* ex;       // the slab's x factors (copied in the set-up round)
 };
 
 // g staging: a wave loads 16 rows x 64 z with consecutive lanes on
@@ -146,7 +146,7 @@ struct FitMfmaLds {
 // the MFMA A operands (8 consecutive z per lane) are then 16-B LDS reads.
 constexpr int kGStride = 72;  // halves per staged row (144 B: rows stay 16-B aligned)
 
-HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
+HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
     Carve c{base, 0};
     FitMfmaLds L;
     L.E = c.take<float>(nbins);
@@ -165,87 +165,80 @@ HD size_t carve_fit_mfma(unsigned char* base, int nbins, int Y, int Z, int MT, i
     return align16(c.off);
 }
 
-template <int PREC, int MT, int NT>
-__device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level, int it, int mode, RunParams rp,
-                              int bid, unsigned char* smem) {
-    const Slab sl = slabs[bid];
-    const VolDesc& V = vols[sl.vol];
-    const VolState* st = V.st;
-    if (!fit_step_mode(st, &mode, &level, &it)) return;
-    const LevelDesc& LD = V.lev[level];
-    const int Y = V.W[1], Z = V.W[2];
+// Thread t's share of the fit's BF16 factor tables: Pz^T over the z extent
+// (NT*16 rows) and Py^T over rows [ylo, ylo + ny) of the y axis (MT*16
+// rows, columns tile-local); kf 2: w^2 (denominator), else w^3/sum w^2.
+// One entry per thread: consecutive threads fill consecutive positions of
+// a control point's row.
+template <int MT, int NT>
+__device__ void fit_mfma_tables_t(const LevelDesc& LD, const FitMfmaLds& S, int Z, int ylo, int ny, int kf, bool pz,
+                                  bool py, int t) {
     const int Cy = LD.C[1], Cz = LD.C[2];
-    const int nb = rp.nbins;
-    FitMfmaLds S;
-    carve_fit_mfma(smem, nb, V.wtile_rows, Z, MT, NT, PREC, &S);
-    const bool num = (mode == kFitNum);
-    const bool den = (mode == kFitDen || mode == kFitDenScatter);
-    const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
-    const int gmode = den ? (int)kFitDen : mode;
-    const bool degenerate = num && st->degenerate;
-    const double bmin = st->bin_min, slope = st->slope;
-    const int kf = den ? 2 : 1;  // q = w^2 for the denominator, p = w^3/S otherwise
+    const int Zp = round_up(Z, 32), Yp = round_up(ny, 16);
+    if (pz)
+        for (int e = t; e < NT * 16 * Zp; e += kBlock) {
+            const int c = e / Zp, z = e - c * Zp;
+            float v = 0.0f;
+            if (z < Z && c < Cz) {
+                const AxisEntry& a = LD.ax[2][z];
+                const int k = c - a.b;
+                if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
+            }
+            S.Pz.put(c, z, v);
+        }
+    if (py)
+        for (int e = t; e < MT * 16 * Yp; e += kBlock) {
+            const int c = e / Yp, y = e - c * Yp;
+            float v = 0.0f;
+            if (y < ny && c < Cy) {
+                const AxisEntry& a = LD.ax[1][ylo + y];
+                const int k = c - a.b;
+                if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
+            }
+            S.Py.put(c, y, v);
+        }
+}
+
+// The voxel block (16 tile-local rows from y0, 64 z from zc) of one wave:
+// consecutive lanes on consecutive z (256 B per row).
+__device__ inline void fit_mfma_load(const VolDesc& V, const Slab sl, int y0, int zc, bool den, uint8_t (&ms)[16],
+                                     float (&xs)[16]) {
+    const int Y = V.W[1], Z = V.W[2];
+    const int lane = (int)threadIdx.x & 63;
+    const int z = zc + lane;
+    UNROLL for (int r = 0; r < 16; ++r) {
+        const bool ok = z < Z && y0 + r < sl.ny;
+        const size_t v = ((size_t)sl.x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
+        ms[r] = ok ? gp(V.wmask)[v] : 0;
+        xs[r] = (ok && !den) ? gp(V.logU)[v] : 0.0f;
+    }
+}
+
+// The (y, z) contraction of one tile on MFMA (tables, E for NUM, and the
+// slab's x factors in S.ex ready; ms/xs hold this wave's first block,
+// fit_mfma_load(16 * wave, 0)), then the scatter (or the slab partial for
+// stage calls).
+template <int PREC, int MT, int NT>
+__device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab sl, const FitMfmaLds& S, int gmode,
+                              bool den, bool scatter, bool degenerate, const Remap& rm, uint8_t (&ms)[16],
+                              float (&xs)[16]) {
+    const int Z = V.W[2];
+    const int Cy = LD.C[1], Cz = LD.C[2];
     const int x = sl.x;
     const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6, r16 = lane & 15, g4 = lane >> 4;
-    const int Yt = sl.ny;  // the tile's rows sl.y0 .. sl.y0 + Yt (Py rows are tile-local)
-    const int Zp = round_up(Z, 32), Yp = round_up(Yt, 16);
-
-    const Remap rm = make_remap(S.E, bmin, slope, nb);
-    // the first block of voxels (row batch 0, z chunk 0) is loaded before
-    // the table set-up, so the two global round trips overlap
-    uint8_t ms[16];
-    float xs[16];
-    auto load_rows = [&](int y0, int zc) {
-        const int z = zc + lane;
-        UNROLL for (int r = 0; r < 16; ++r) {
-            const bool ok = z < Z && y0 + r < Yt;
-            const size_t v = ((size_t)x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
-            ms[r] = ok ? gp(V.wmask)[v] : 0;
-            xs[r] = (ok && !den) ? gp(V.logU)[v] : 0.0f;
-        }
-    };
-    load_rows(16 * wv, 0);
-    if (num)
-        for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
-    // factor tables, one entry per thread: consecutive threads fill
-    // consecutive positions of a control point's row
-    for (int e = t; e < NT * 16 * Zp; e += kBlock) {
-        const int c = e / Zp, z = e - c * Zp;
-        float v = 0.0f;
-        if (z < Z && c < Cz) {
-            const AxisEntry& a = LD.ax[2][z];
-            const int k = c - a.b;
-            if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
-        }
-        S.Pz.put(c, z, v);
-    }
-    for (int e = t; e < MT * 16 * Yp; e += kBlock) {
-        const int c = e / Yp, y = e - c * Yp;
-        float v = 0.0f;
-        if (y < Yt && c < Cy) {
-            const AxisEntry& a = LD.ax[1][sl.y0 + y];
-            const int k = c - a.b;
-            if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
-        }
-        S.Py.put(c, y, v);
-    }
-    // the slab's x factors for the final scatter, in this round of loads
-    if (t < (int)(sizeof(AxisEntry) / 4))
-        reinterpret_cast<int32_t*>(S.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][x])[t];
-    __syncthreads();
-
+    const int Yt = sl.ny;  // the tile's rows sl.y0 .. sl.y0 + Yt (Py columns are tile-local)
     f32x4 acc[MT][NT];
     UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0, 0, 0, 0};
 
-    unsigned short* Gh = S.Gh + wv * 16 * kGStride;
-    unsigned short* Gl = (PREC == 2) ? S.Gl + wv * 16 * kGStride : nullptr;
+    N4_LDS unsigned short* Gh = S.Gh + wv * 16 * kGStride;
+    N4_LDS unsigned short* Gl = (PREC == 2) ? S.Gl + wv * 16 * kGStride : nullptr;
     // every wave runs the same trip counts (barriers inside)
     for (int rb = 0; rb < Yt; rb += 64) {
         const int y0 = rb + 16 * wv;  // this wave's 16 tile-local rows (past Yt: zeros)
         f32x4 t1[NT];
         UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
         for (int zc = 0; zc < Z; zc += 64) {
-            if (rb > 0 || zc > 0) load_rows(y0, zc);  // (the first block is already in registers)
+            if (rb > 0 || zc > 0) fit_mfma_load(V, sl, y0, zc, den, ms, xs);  // (the first block is already in registers)
             UNROLL for (int r = 0; r < 16; ++r) {
                 const float gv = fit_input(gmode, ms[r], xs[r], degenerate, rm);
                 unsigned short h, l = 0;
@@ -260,9 +253,9 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
             UNROLL for (int ks = 0; ks < 2; ++ks) {
                 const int z0 = zc + 32 * ks;
                 if (z0 < Z) {
-                    const u16x8 ah = *reinterpret_cast<const u16x8*>(Gh + r16 * kGStride + 32 * ks + 8 * g4);
+                    const u16x8 ah = *reinterpret_cast<const N4_LDS u16x8*>(Gh + r16 * kGStride + 32 * ks + 8 * g4);
                     const u16x8 al =
-                        (PREC == 2) ? *reinterpret_cast<const u16x8*>(Gl + r16 * kGStride + 32 * ks + 8 * g4) : ah;
+                        (PREC == 2) ? *reinterpret_cast<const N4_LDS u16x8*>(Gl + r16 * kGStride + 32 * ks + 8 * g4) : ah;
                     UNROLL for (int n = 0; n < NT; ++n) {
                         const u16x8 bh = S.Pz.get8(S.Pz.hi, n * 16 + r16, z0 + 8 * g4);
                         const u16x8 bl = S.Pz.get8(S.Pz.lo, n * 16 + r16, z0 + 8 * g4);
@@ -309,6 +302,42 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     }
 }
 
+template <int PREC, int MT, int NT>
+__device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level, int it, int mode, RunParams rp,
+                              int bid, N4_LDS unsigned char* smem) {
+    const Slab sl = slabs[bid];
+    const VolDesc& V = vols[sl.vol];
+    const VolState* st = V.st;
+    if (!fit_step_mode(st, &mode, &level, &it)) return;
+    const LevelDesc& LD = V.lev[level];
+    const int Z = V.W[2];
+    const int nb = rp.nbins;
+    FitMfmaLds S;
+    carve_fit_mfma(smem, nb, V.wtile_rows, Z, MT, NT, PREC, &S);
+    const bool num = (mode == kFitNum);
+    const bool den = (mode == kFitDen || mode == kFitDenScatter);
+    const bool scatter = (mode == kFitNum || mode == kFitDenScatter);
+    const int gmode = den ? (int)kFitDen : mode;
+    const bool degenerate = num && st->degenerate;
+    const double bmin = st->bin_min, slope = st->slope;
+    const int kf = den ? 2 : 1;  // q = w^2 for the denominator, p = w^3/S otherwise
+    const int t = (int)threadIdx.x, wv = t >> 6;
+    const Remap rm = make_remap(S.E, bmin, slope, nb);
+    // the first block of voxels (row batch 0, z chunk 0) is loaded before
+    // the table set-up, so the two global round trips overlap
+    uint8_t ms[16];
+    float xs[16];
+    fit_mfma_load(V, sl, 16 * wv, 0, den, ms, xs);
+    if (num)
+        for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
+    fit_mfma_tables_t<MT, NT>(LD, S, Z, sl.y0, sl.ny, kf, true, true, t);
+    // the slab's x factors for the final scatter, in this round of loads
+    if (t < (int)(sizeof(AxisEntry) / 4))
+        reinterpret_cast<N4_LDS int32_t*>(S.ex)[t] = reinterpret_cast<const int32_t*>(&LD.ax[0][sl.x])[t];
+    __syncthreads();
+    fit_mfma_tile<PREC, MT, NT>(V, LD, sl, S, gmode, den, scatter, degenerate, rm, ms, xs);
+}
+
 // ------------------------------------------------------------------ eval --
 // Working-grid evaluation of the lattice increment (logU update and CV
 // sums).  Four control planes along x feed each slab: A[cy,cz] =
@@ -317,14 +346,14 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
 // HBM and stays on the FP32 kernel in every mode: an MFMA form with its
 // BF16 tables in LDS ran 5x slower there, limited by occupancy.)
 struct EvalMfmaLds {
-    float* A;    // [Cy*Cz]     x-contracted planes
+    N4_LDS float* A;    // [Cy*Cz]     x-contracted planes
     Tab At;      // [NT*16][MT*16 + 4]  A^T: row = cz, column = cy
     Tab Wy;      // [Yrows][MT*16 + 4]  row = y (slab-local), column = cy
     Tab Wz;      // [Zp16][NT*16 + 4]   row = z, column = cz
-    double* red; // [4*kBlock]
+    N4_LDS double* red; // [4*kBlock]
 };
 
-HD size_t carve_eval_mfma(unsigned char* base, int Yrows, int Z, int Cy, int Cz, int MT, int NT, int prec,
+HD size_t carve_eval_mfma(N4_LDS unsigned char* base, int Yrows, int Z, int Cy, int Cz, int MT, int NT, int prec,
                           EvalMfmaLds* o) {
     Carve c{base, 0};
     EvalMfmaLds L;
@@ -404,7 +433,7 @@ __device__ inline f32x4 eval_mfma_tile(const EvalMfmaLds& S, int z0, int r16, in
 // Field increment d on the working grid, logU -= d, CV sums and logU
 // min/max per slab (same contract as the FP32 evaluation kernel).
 template <int PREC, int MT, int NT>
-__device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, unsigned char* smem) {
+__device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
     if (V.st->s_kind != kStepIter) return;

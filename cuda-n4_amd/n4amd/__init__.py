@@ -37,7 +37,7 @@ EXPORTS = [
     "n4_memcpy_dtoh", "n4_memcpy_dtod", "n4_memset", "n4_synchronize", "n4_profile_enable",
     "n4_profile_read", "n4_get_tuning", "n4_set_tuning", "n4_autotune_batch",
 ]
-KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice", "update"]
+KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice", "update", "teams"]
 
 
 class Params(C.Structure):
@@ -86,6 +86,8 @@ class Volume(C.Structure):
 GRAPH_OFF = 0xFFFFFFFF
 EVAL_KERNELS = {"fp32": 1, "mfma": 2}
 EVAL_KERNELS_INV = {v: k for k, v in EVAL_KERNELS.items()}
+PATHS = {"auto": 0, "steps": 1, "teams": 2}
+PATHS_INV = {v: k for k, v in PATHS.items()}
 
 
 class Tuning(C.Structure):
@@ -95,13 +97,17 @@ class Tuning(C.Structure):
         ("step_tile_voxels", C.c_uint32),
         ("graph_steps", C.c_uint32),
         ("eval_kernel", C.c_uint32),
+        ("path", C.c_uint32),
+        ("team_wgs_per_cu", C.c_uint32),
         ("ms", C.c_float),
     ]
 
     def as_dict(self):
         return {"groups": int(self.groups), "step_tile_voxels": int(self.step_tile_voxels),
                 "graph_steps": "off" if self.graph_steps == GRAPH_OFF else int(self.graph_steps),
-                "eval_kernel": EVAL_KERNELS_INV.get(int(self.eval_kernel), 0), "ms": float(self.ms)}
+                "eval_kernel": EVAL_KERNELS_INV.get(int(self.eval_kernel), 0),
+                "path": PATHS_INV.get(int(self.path), 0), "team_wgs_per_cu": int(self.team_wgs_per_cu),
+                "ms": float(self.ms)}
 
 
 class N4Error(RuntimeError):
@@ -304,15 +310,18 @@ class Library:
         self._check(self.L.n4_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, groups=0, step_tile_voxels=0, graph_steps=0, eval_kernel=0, reset=False):
+    def set_tuning(self, groups=0, step_tile_voxels=0, graph_steps=0, eval_kernel=0, path=0, team_wgs_per_cu=0,
+                   reset=False, ms=None):
         """graph_steps: 0 default, "off" (or GRAPH_OFF) direct launches;
-        eval_kernel: 0 default, "fp32" or "mfma"."""
+        eval_kernel: 0 default, "fp32" or "mfma"; path: 0/"auto", "steps" or
+        "teams"."""
         if reset:
             self._check(self.L.n4_set_tuning(self.h, None))
             return
         g = GRAPH_OFF if graph_steps == "off" else int(graph_steps)
         e = EVAL_KERNELS[eval_kernel] if isinstance(eval_kernel, str) else int(eval_kernel)
-        t = Tuning(int(groups), int(step_tile_voxels), g, e, 0.0)
+        pth = PATHS[path] if isinstance(path, str) else int(path)
+        t = Tuning(int(groups), int(step_tile_voxels), g, e, pth, int(team_wgs_per_cu), 0.0)
         self._check(self.L.n4_set_tuning(self.h, C.byref(t)))
 
     def autotune_batch(self, vols, params: Params | None = None, shrink=None, **kw) -> Tuning:

@@ -178,9 +178,17 @@ n4_status n4_reduce_max(n4_handle *h, uint64_t n, const float *d_in, float *out)
 n4_status n4_reduce_min(n4_handle *h, uint64_t n, const float *d_in, float *out);
 
 /* ---- launch schedule and autotuner (no counterpart in the reference) --- */
-/* How n4_correct / n4_correct_batch lay out their launches.  Results do not
- * depend on the schedule except through the summation order of the FP64
- * convergence partials (one partial per tile).  A field left 0 takes the
+/* How n4_correct / n4_correct_batch lay out their launches.  Two paths run
+ * the level/iteration loop (cudaN4.cu:1517-1657):
+ *   steps: the host enqueues identical steps of five launches (histogram,
+ *     Wiener, fit, lattice update, evaluation) per volume group, each
+ *     volume advancing its own state machine on the device;
+ *   teams: one persistent launch in which a team of workgroups per volume
+ *     runs the volume's whole loop, synchronising at team barriers (every
+ *     workgroup of the launch is resident at once).
+ * Both run the same per-tile arithmetic, so they give the same bits.
+ * Results do not depend on the schedule except through the summation order
+ * of the FP64 convergence partials (one partial per tile).  A field left 0 takes the
  * environment override (N4_GROUPS, N4_STEP_TILE_VOX, N4_GRAPH_STEPS) if set,
  * else the built-in default. */
 #define N4_GRAPH_OFF 0xFFFFFFFFu
@@ -194,10 +202,19 @@ typedef struct n4_tuning {
     uint32_t eval_kernel;      /* field-increment kernel of the MFMA
                                   precisions: N4_EVAL_FP32 (default) or
                                   N4_EVAL_MFMA; FP32 runs use FP32 */
+    uint32_t path;             /* N4_PATH_AUTO (default): the persistent team
+                                  kernel where the batch is eligible, else the
+                                  step kernels; N4_PATH_STEPS; N4_PATH_TEAMS
+                                  (N4_ERR_UNSUPPORTED where not eligible) */
+    uint32_t team_wgs_per_cu;  /* persistent path: workgroups per compute
+                                  unit (0 = as many as fit, at most 4) */
     float ms;                  /* n4_autotune_batch: wall time of one batch */
 } n4_tuning;
 #define N4_EVAL_FP32 1u
 #define N4_EVAL_MFMA 2u
+#define N4_PATH_AUTO 0u
+#define N4_PATH_STEPS 1u
+#define N4_PATH_TEAMS 2u
 /* The handle's schedule as set (0 fields = defaults). */
 n4_status n4_get_tuning(n4_handle *h, n4_tuning *out);
 /* Sets the handle's schedule; NULL restores the defaults. */
@@ -223,7 +240,8 @@ typedef enum n4_kernel_id {
     N4_K_WIENER = 7,  /* Wiener deconvolution + E table (one block/volume) */
     N4_K_LATTICE = 8, /* mode-x contraction + lattice update (stage calls) */
     N4_K_UPDATE = 9,  /* lattice update of a correction step */
-    N4_K_COUNT = 10
+    N4_K_TEAMS = 10,  /* persistent path: every volume's level/iteration loop */
+    N4_K_COUNT = 11
 } n4_kernel_id;
 /* Time every launch of the kernels in kernel_mask (bit = n4_kernel_id) with
  * a pair of HIP events recorded on the launch stream around it (step graphs
