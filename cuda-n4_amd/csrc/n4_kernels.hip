@@ -175,20 +175,34 @@ HD int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // (exact while v * d < 2^40).
 HD uint32_t fdiv(uint32_t v, uint64_t m) { return (uint32_t)(((uint64_t)v * m) >> kDivShift); }
 
-// Bump allocator over the block's dynamic LDS; with base == nullptr it only
-// measures (host-side sizing uses the same code as the device carve).  Every
-// carve-out is typed as LDS (N4_LDS), so the compiler rejects passing one
-// where a generic pointer is expected: LDS is only ever read and written
-// with ds_* instructions, never with flat ones.
+// Bump allocator over the block's dynamic LDS; on the host, with base ==
+// nullptr, it only measures (host-side sizing uses the same code as the
+// device carve).  Every carve-out is typed as LDS (N4_LDS), so the compiler
+// rejects passing one where a generic pointer is expected: LDS is only ever
+// read and written with ds_* instructions, never with flat ones.  On the
+// device an LDS pointer is never tested against null: the dynamic LDS
+// starts at offset 0, which compares equal to the LDS null pointer.
 struct Carve {
     N4_LDS unsigned char* base;
     size_t off;
     template <class T>
     HD N4_LDS T* take(size_t n) {
         off = align16(off);
+#ifdef N4_DEVICE_PASS
+        N4_LDS T* p = reinterpret_cast<N4_LDS T*>(base + off);
+#else
         N4_LDS T* p = base ? reinterpret_cast<N4_LDS T*>(base + off) : nullptr;
+#endif
         off += n * sizeof(T);
         return p;
+    }
+    // the region at offset o of the same block (measuring: nullptr)
+    HD N4_LDS unsigned char* at(size_t o) const {
+#ifdef N4_DEVICE_PASS
+        return base + o;
+#else
+        return base ? base + o : nullptr;
+#endif
     }
 };
 
@@ -1950,6 +1964,7 @@ HD size_t carve_persist(N4_LDS unsigned char* base, const VolDesc& V, int nbins,
         L.M.ex = L.F.ex;
         const int zs = round_up(Z, 32) + 8;
         L.M.Pz.stride = zs;
+        L.M.Pz.has_lo = true;
         L.M.Pz.hi = c.take<unsigned short>((size_t)16 * zs);
         L.M.Pz.lo = c.take<unsigned short>((size_t)16 * zs);
     }
@@ -1961,7 +1976,7 @@ HD size_t carve_persist(N4_LDS unsigned char* base, const VolDesc& V, int nbins,
     L.Ev.red = L.red;
     c.off = align16(c.off);
     const size_t u0 = c.off;
-    L.u = base ? base + u0 : nullptr;
+    L.u = c.at(u0);
     size_t um = 0;
     {  // histogram
         Carve h{L.u, 0};
@@ -1985,6 +2000,7 @@ HD size_t carve_persist(N4_LDS unsigned char* base, const VolDesc& V, int nbins,
         Carve f{L.u, 0};
         const int ys = round_up(V.wtile_rows, 16) + 4;
         L.M.Py.stride = ys;
+        L.M.Py.has_lo = true;
         L.M.Py.hi = f.take<unsigned short>((size_t)16 * ys);
         L.M.Py.lo = f.take<unsigned short>((size_t)16 * ys);
         L.M.red = f.take<float>(256);

@@ -85,16 +85,19 @@ __device__ inline void to_op4(f32x4 v, i16x4& hi, i16x4& lo) {
     }
 }
 
-// BF16 factor table in LDS: rows of `stride` halves, hi then (PREC 2) lo.
+// BF16 factor table in LDS: rows of `stride` halves, hi and (has_lo) lo.
+// (An LDS pointer is never tested against null: offset 0 is a valid
+// carve-out and compares equal to the LDS null pointer.)
 struct Tab {
     N4_LDS unsigned short* hi;
     N4_LDS unsigned short* lo;
     int stride;
+    bool has_lo;
     __device__ void put(int r, int c, float v) const {
         unsigned short h, l;
         bf16_split(v, h, l);
         hi[r * stride + c] = h;
-        if (lo) lo[r * stride + c] = l;
+        if (has_lo) lo[r * stride + c] = l;
     }
     __device__ i16x4 get4(const N4_LDS unsigned short* t, int r, int c) const {
         return *reinterpret_cast<const N4_LDS i16x4*>(t + r * stride + c);
@@ -152,9 +155,11 @@ HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, in
     L.E = c.take<float>(nbins);
     const int zs = round_up(Z, 32) + 8, ys = round_up(Y, 16) + 4;
     L.Pz.stride = zs;
+    L.Pz.has_lo = true;
     L.Pz.hi = c.take<unsigned short>((size_t)NT * 16 * zs);
     L.Pz.lo = c.take<unsigned short>((size_t)NT * 16 * zs);
     L.Py.stride = ys;
+    L.Py.has_lo = true;
     L.Py.hi = c.take<unsigned short>((size_t)MT * 16 * ys);
     L.Py.lo = c.take<unsigned short>((size_t)MT * 16 * ys);
     L.red = c.take<float>((size_t)MT * NT * 256);
@@ -360,14 +365,17 @@ HD size_t carve_eval_mfma(N4_LDS unsigned char* base, int Yrows, int Z, int Cy, 
     L.A = c.take<float>((size_t)Cy * Cz);
     const int as = MT * 16 + 4, zs = NT * 16 + 4;
     L.At.stride = as;
+    L.At.has_lo = prec == 2;
     L.At.hi = c.take<unsigned short>((size_t)NT * 16 * as);
     L.At.lo = prec == 2 ? c.take<unsigned short>((size_t)NT * 16 * as) : nullptr;
     const int yr = round_up(Yrows, 16);
     L.Wy.stride = as;
+    L.Wy.has_lo = true;
     L.Wy.hi = c.take<unsigned short>((size_t)yr * as);
     L.Wy.lo = c.take<unsigned short>((size_t)yr * as);
     const int zr = round_up(Z, 16);
     L.Wz.stride = zs;
+    L.Wz.has_lo = true;
     L.Wz.hi = c.take<unsigned short>((size_t)zr * zs);
     L.Wz.lo = c.take<unsigned short>((size_t)zr * zs);
     L.red = c.take<double>((size_t)4 * kBlock);
