@@ -374,10 +374,10 @@ def main():
         # the dominant kernel, then the same number of steps as timed with
         # events around each of its launches give its average duration
         lib.set_tuning(groups=1, graph_steps="off")  # library defaults otherwise
-        lib.profile_enable(n4amd.KERNELS)
+        lib.profile_enable(n4amd.LAUNCH_KERNELS)
         batch_step(p)
         lib.synchronize()
-        breakdown = {k: lib.profile_read(k) for k in n4amd.KERNELS}
+        breakdown = {k: lib.profile_read(k) for k in n4amd.LAUNCH_KERNELS}
         dominant = max(breakdown, key=lambda k: breakdown[k][0])
         log(f"[{prec}] one-group per-kernel device time (ms, launches): "
             + ", ".join(f"{k}={v[0]:.3f}/{v[1]}" for k, v in breakdown.items()))

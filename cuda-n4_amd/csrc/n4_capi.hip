@@ -326,6 +326,10 @@ static Schedule resolve_schedule(const n4_handle* h, int nvol) {
                                             : (uint32_t)env_int("N4_EVAL_KERNEL", (int)N4_EVAL_FP32);
     s.mfma_eval = ek == N4_EVAL_MFMA;
     s.path = h->tune.path ? (int)h->tune.path : env_int("N4_PATH", (int)N4_PATH_AUTO);
+    // the step kernels measured faster than the team kernel on every
+    // workload tried (per-tile memory latency is exposed when a workgroup
+    // walks its tiles one after another), so AUTO runs the steps
+    if (s.path == (int)N4_PATH_AUTO) s.path = (int)N4_PATH_STEPS;
     s.team_per_cu = h->tune.team_wgs_per_cu ? (int)h->tune.team_wgs_per_cu : env_int("N4_TEAM_WGS_PER_CU", 0);
     return s;
 }
@@ -350,6 +354,7 @@ static int rows_for(int rows, int cols, int target) {
 // the descriptors are arena-relative until `rebase` adds the arena base.
 // tiled: cut working slabs into row tiles for the step kernels (the
 // correction); stage calls keep whole slabs (their partials are per slab).
+
 // teams (nullable, persistent path): workgroups per volume; the plan then
 // holds the PWork list (consecutive ranks per volume, each rank a
 // contiguous share of the volume's tiles), a barrier slot per volume and
@@ -450,7 +455,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.spart = lay.take(sizeof(double) * 4 * I.W[0] * I.wtiles_x);
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
         o.etab = lay.take(sizeof(double) * nb);
-        o.team = teams ? lay.take(64) : 0;
+        o.team = teams ? lay.take(64 + sizeof(uint64_t) * kPhases) : 0;
     }
     P->total_bytes = lay.take(0);
 
@@ -514,6 +519,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.wtiles_x = I.wtiles_x;
         D.hist_copies = hcopies(v);
         D.team_ctr = teams ? (uint32_t*)(base + o.team) : nullptr;
+        D.tphase = (teams && (h->prof_mask >> N4_K_TEAMS & 1u)) ? (uint64_t*)(base + o.team + 64) : nullptr;
         D.mW2 = magic_div((uint32_t)I.W[2]);
         D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
@@ -1182,8 +1188,12 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     std::vector<VolState> st(nv);
     D2H(st.data(), dev_state(h, P), sizeof(VolState) * nv, stream);
     std::vector<uint32_t> tctr(teams_path ? 2 * nv : 0);
+    std::vector<uint64_t> tph(teams_path && P.desc[0].tphase ? (size_t)kPhases * nv : 0);
     if (teams_path)
-        for (int v = 0; v < nv; ++v) D2H(&tctr[2 * v], P.desc[v].team_ctr, 2 * sizeof(uint32_t), stream);
+        for (int v = 0; v < nv; ++v) {
+            D2H(&tctr[2 * v], P.desc[v].team_ctr, 2 * sizeof(uint32_t), stream);
+            if (!tph.empty()) D2H(&tph[(size_t)kPhases * v], P.desc[v].tphase, sizeof(uint64_t) * kPhases, stream);
+        }
     float ms = 0.0f;
 #ifndef N4_EMULATE
     N4_CHECK_HIP(hipEventRecord(ev.b, (hipStream_t)stream));
@@ -1192,6 +1202,16 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     N4_CHECK_HIP(hipGetLastError());
 #endif
     prof_collect(h);
+    if (!tph.empty()) {
+        // the leaders' phase clocks: total over the volumes, launches = steps
+        static const int kPhaseId[kPhSteps] = {N4_K_T_WAIT, N4_K_T_REDUCE, N4_K_T_HIST, N4_K_T_WIENER,
+                                               N4_K_T_FIT, N4_K_T_UPDATE, N4_K_T_EVAL, N4_K_T_TRANS};
+        for (int v = 0; v < nv; ++v)
+            for (int k = 0; k < kPhSteps; ++k) {
+                h->prof_ms[kPhaseId[k]] += 1e3 * (double)tph[(size_t)kPhases * v + k] / kClockHz;
+                h->prof_n[kPhaseId[k]] += tph[(size_t)kPhases * v + kPhSteps];
+            }
+    }
     for (int v = 0; v < (int)tctr.size() / 2; ++v)
         if (tctr[2 * v + 1])
             return fail(N4_ERR_INTERNAL, "persistent path: a team barrier of volume %d timed out", order[v]);

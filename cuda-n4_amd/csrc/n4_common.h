@@ -122,7 +122,14 @@ struct VolDesc {
     uint64_t mW2;              // magic divisor for the working z extent
     uint64_t mF2;              // magic divisor for the full-resolution z extent
     uint32_t* team_ctr;        // persistent path: [0] the team's barrier counter, [1] abort flag
+    uint64_t* tphase;          // persistent path (nullable): leader's device-clock ticks per phase (kPhases)
 };
+
+// Phases of the persistent path timed by a team's leader (tphase slots).
+enum TeamPhase : int32_t {
+    kPhWait = 0, kPhReduce, kPhHist, kPhWiener, kPhFit, kPhUpdate, kPhEval, kPhTrans, kPhSteps, kPhases
+};
+constexpr double kClockHz = 100.0e6;  // s_memrealtime
 
 struct D2 {
     double x, y;

@@ -51,28 +51,6 @@
 #define UNROLL
 #endif
 
-// Work of wavefront 0 alone (lane l of 64), with wave-level ordering of its
-// LDS phases only (no workgroup barrier): a wave's LDS instructions execute
-// in issue order, so a phase sees the previous phase's stores.
-#ifdef N4_DEVICE_PASS
-#define WAVE0_BEGIN if (threadIdx.x < 64) {
-#define WAVE0_END }
-#define WAVE_FOR(l) { const int l = (int)threadIdx.x; (void)l;
-#define WAVE_END }
-#define WAVE_SYNC()                                         \
-    do {                                                    \
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
-        __builtin_amdgcn_wave_barrier();                    \
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
-    } while (0)
-#else
-#define WAVE0_BEGIN {
-#define WAVE0_END }
-#define WAVE_FOR(l) for (int l = 0; l < 64; ++l) {
-#define WAVE_END }
-#define WAVE_SYNC() do { } while (0)
-#endif
-
 #define HD __host__ __device__ inline
 
 // LDS view: every pointer into a workgroup's LDS carries this address space.
@@ -610,6 +588,7 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, N4_LDS
     for (int l = t; l < Lmax * V.num_copies; l += kBlock) V.numfx[l] = 0;
     for (int i = t; i < rp.nbins * V.hist_copies; i += kBlock) V.hist[i] = 0;
     if (t < 2 && V.team_ctr) V.team_ctr[t] = 0;
+    if (t < kPhases && V.tphase) V.tphase[t] = 0;
     if (t == 0) {
         st->n_mask_full = (uint64_t)red[0];
         st->n_mask_work = (uint64_t)red[kBlock];
@@ -630,6 +609,13 @@ HD void body_level(const VolDesc* vols, int level, RunParams rp, int bid, N4_LDS
     }
     BLOCK_END
 }
+
+// Where the histogram copies come from and the table goes:
+//   kWienerLocal  the copies were completed by earlier launches: plain
+//                 loads, cleared; table to V.etab
+//   kWienerTeam   persistent path: agent-scope loads, left for the team's
+//                 leader to clear; table to Ef (FP32, LDS: the remap's copy)
+enum WienerMode : int { kWienerLocal = 0, kWienerTeam = 2 };
 
 // ------------------------------------------------------------- histogram --
 // Fractional histogram of one tile into kHistLanes lane-strided LDS copies
@@ -788,13 +774,14 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
         }
         BLOCK_END
     }
-    if (kind != kStepIter || degenerate) return;
-    hist_zero(h, hs);
-    BLOCK_SYNC();
-    hist_tile(V, sl, mn, slope, nb, h, hs, dbg_idx);
-    BLOCK_SYNC();
-    // tiles spread their flush over the copies: fewer atomics per address
-    hist_flush(h, hs, nb, V.hist + (size_t)(tile_id(V, sl) % V.hist_copies) * nb);
+    if (kind == kStepIter && !degenerate) {
+        hist_zero(h, hs);
+        BLOCK_SYNC();
+        hist_tile(V, sl, mn, slope, nb, h, hs, dbg_idx);
+        BLOCK_SYNC();
+        // tiles spread their flush over the copies: fewer atomics per address
+        hist_flush(h, hs, nb, V.hist + (size_t)(tile_id(V, sl) % V.hist_copies) * nb);
+    }
 }
 
 // ---------------------------------------------------------------- Wiener --
@@ -843,15 +830,15 @@ HD void dft_small(D2 (&v)[R], double sg) {
 
 // One Stockham pass of radix R over P points (span Ns): butterfly j takes
 // a[j + r*P/R], twiddles them by exp(-+2 pi i r k / (Ns R)) (k = j mod Ns)
-// and writes b[(j - k) R + k + r Ns].
-// The same pass run by one wavefront (lane l of 64).
+// and writes b[(j - k) R + k + r Ns].  Thread t of the block takes
+// butterflies t, t + kBlock, ...
 template <int R>
-HD void wave_fft_pass(const N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, int Ns, bool inverse) {
+HD void block_fft_pass(const N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, int Ns, bool inverse) {
     const int nbf = P / R;
     const int tstep = P / (Ns * R);
     const double sg = inverse ? 1.0 : -1.0;
-    WAVE_FOR(l)
-    for (int j = l; j < nbf; j += 64) {
+    BLOCK_FOR(t)
+    for (int j = t; j < nbf; j += kBlock) {
         const int k = j & (Ns - 1);
         D2 v[R];
         UNROLL for (int r = 0; r < R; ++r) v[r] = a[j + r * nbf];
@@ -866,24 +853,27 @@ HD void wave_fft_pass(const N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int
         const int o = (j - k) * R + k;
         UNROLL for (int r = 0; r < R; ++r) b[o + r * Ns] = v[r];
     }
-    WAVE_END
+    BLOCK_END
 }
 
-// LDS FP64 Stockham FFT over P = 2^k points with radix-8 passes (radix 4
-// or 2 for the last one); tw[m] = exp(-2 pi i m / P), m < P, in LDS.  Run by
-// wavefront 0 alone.  Returns the buffer holding the result (the other one
-// is free).
-HD N4_LDS D2* wave_fft(N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, bool inverse) {
+// LDS FP64 Stockham FFT over P = 2^k points (tw[m] = exp(-2 pi i m / P), m <
+// P, in LDS): radix-8 passes (radix 4 or 2 for the last), a block barrier
+// between passes.  Returns the buffer holding the result (the other one is
+// free).  (Radix 2 and 4 over all 256 threads, and radix 8 in one
+// wavefront with wave-level ordering, measured equal or slower on MI355X:
+// the Wiener kernel's time is mostly its gather and launch, not the chain.)
+constexpr int kFftRadix = 8;
+HD N4_LDS D2* block_fft(N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, bool inverse) {
     for (int Ns = 1; Ns < P;) {
         const int rem = P / Ns;
-        const int R = rem >= 8 ? 8 : rem;
+        const int R = rem >= kFftRadix ? kFftRadix : rem;
         if (R == 8)
-            wave_fft_pass<8>(a, b, tw, P, Ns, inverse);
+            block_fft_pass<8>(a, b, tw, P, Ns, inverse);
         else if (R == 4)
-            wave_fft_pass<4>(a, b, tw, P, Ns, inverse);
+            block_fft_pass<4>(a, b, tw, P, Ns, inverse);
         else
-            wave_fft_pass<2>(a, b, tw, P, Ns, inverse);
-        WAVE_SYNC();
+            block_fft_pass<2>(a, b, tw, P, Ns, inverse);
+        BLOCK_SYNC();
         N4_LDS D2* tmp = a;
         a = b;
         b = tmp;
@@ -896,12 +886,11 @@ HD N4_LDS D2* wave_fft(N4_LDS D2* a, N4_LDS D2* b, const N4_LDS D2* tw, int P, b
 // N4.py:31-101 (cudaN4.cu:733-820 did this on the host with FFTW).  Two real
 // transforms share one complex FFT (V in the real part, F in the
 // imaginary).  Reads and clears the volume's histogram.
-// TEAM (persistent path): the copies are read with agent-scope loads and
-// left for the team's leader to clear, and the table goes to Ef (FP32, LDS:
-// the remap's copy) instead of V.etab.
-template <bool TEAM>
+template <int WM>
 HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, N4_LDS unsigned char* smem,
                      N4_LDS float* Ef = nullptr) {
+    constexpr bool TEAM = WM == kWienerTeam;
+    constexpr bool AGENT = WM != kWienerLocal;
     const int P = rp.P, half = P / 2, off = rp.offset, nb = rp.nbins;
     Carve cv{smem, 0};
     N4_LDS D2* z = cv.take<D2>(P);
@@ -922,16 +911,17 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
             const int nc = V.hist_copies;
             uint64_t c[kMaxCopies];
             UNROLL for (int k = 0; k < kMaxCopies; ++k) {
-                if (TEAM)
+                if (AGENT)
                     c[k] = k < nc ? ld_agent(hp + (size_t)k * nb) : 0;
                 else
                     c[k] = k < nc ? hp[(size_t)k * nb] : 0;
             }
             uint64_t s = 0;
             UNROLL for (int k = 0; k < kMaxCopies; ++k) s += c[k];
-            if (!TEAM)
+            if (WM == kWienerLocal)
                 UNROLL for (int k = 0; k < kMaxCopies; ++k)
                     if (k < nc) hp[(size_t)k * nb] = 0;
+
             Vj = (double)s * 0x1p-32;
         }
         z[j] = D2{Vj, Fj};
@@ -939,13 +929,12 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
     }
     BLOCK_END
     BLOCK_SYNC();
-    // the sequential chain of transforms and pointwise steps: wavefront 0
-    WAVE0_BEGIN
-    N4_LDS D2* Z = wave_fft(z, y, tw, P, false);
+    // the chain of transforms and pointwise steps, block-wide
+    N4_LDS D2* Z = block_fft(z, y, tw, P, false);
     N4_LDS D2* U = (Z == z) ? y : z;
     // Split Vf and Ff, Gf = conj(Ff)/(conj(Ff)Ff + noise), Uf = Vf*Re(Gf)
-    WAVE_FOR(l)
-    for (int k = l; k < P; k += 64) {
+    BLOCK_FOR(t)
+    for (int k = t; k < P; k += kBlock) {
         const D2 a = Z[k];
         const D2 m = Z[(P - k) & (P - 1)];
         const D2 Vf{0.5 * (a.x + m.x), 0.5 * (a.y - m.y)};
@@ -954,28 +943,28 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
         const double g = Ff.x / ((Ff.x * Ff.x + Ff.y * Ff.y) + rp.noise);
         U[k] = D2{Vf.x * g, Vf.y * g};
     }
-    WAVE_END
-    WAVE_SYNC();
-    N4_LDS D2* Ur = wave_fft(U, Z, tw, P, true);
+    BLOCK_END
+    BLOCK_SYNC();
+    N4_LDS D2* Ur = block_fft(U, Z, tw, P, true);
     N4_LDS D2* Wb = (Ur == U) ? Z : U;
     const double invP = 1.0 / (double)P;
-    WAVE_FOR(l)
-    for (int j = l; j < P; j += 64) {
+    BLOCK_FOR(t)
+    for (int j = t; j < P; j += kBlock) {
         double u = Ur[j].x * invP;
         if (u < 0.0) u = 0.0;
         Wb[j] = D2{(bmin + (double)(j - off) * slope) * u, u};
     }
-    WAVE_END
-    WAVE_SYNC();
-    N4_LDS D2* Wf = wave_fft(Wb, Ur, tw, P, false);
+    BLOCK_END
+    BLOCK_SYNC();
+    N4_LDS D2* Wf = block_fft(Wb, Ur, tw, P, false);
     N4_LDS D2* Wo = (Wf == Wb) ? Ur : Wb;
-    WAVE_FOR(l)
-    for (int k = l; k < P; k += 64) Wf[k] = cmul(Wf[k], ff[k]);
-    WAVE_END
-    WAVE_SYNC();
-    N4_LDS D2* R = wave_fft(Wf, Wo, tw, P, true);
-    WAVE_FOR(l)
-    for (int i = l; i < nb; i += 64) {
+    BLOCK_FOR(t)
+    for (int k = t; k < P; k += kBlock) Wf[k] = cmul(Wf[k], ff[k]);
+    BLOCK_END
+    BLOCK_SYNC();
+    N4_LDS D2* R = block_fft(Wf, Wo, tw, P, true);
+    BLOCK_FOR(t)
+    for (int i = t; i < nb; i += kBlock) {
         const double nu = R[off + i].x * invP;
         const double de = R[off + i].y * invP;
         const double e = (de != 0.0) ? nu / de : 0.0;
@@ -984,23 +973,20 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
         else
             V.etab[i] = e;
     }
-    WAVE_END
-    WAVE0_END
+    BLOCK_END
 }
 
-// One block per volume: the step's per-volume work and the state for the
-// next step.
+// A volume's per-step work and the state for the next step (one block):
 //   ITER   Wiener step (unless degenerate); next: ITER(l, it+1)
 //   TRANS  refine the lattice into level l (l > 0); next: ITER(l, 0)
 //   FINAL  FP32 copy of the final lattice; next: finished
-HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, N4_LDS unsigned char* smem) {
-    const VolDesc& V = vols[bid];
+// WM: where the histogram copies come from (WienerMode).
+template <int WM>
+HD void volume_step(const VolDesc& V, int kind, int s_lvl, int s_it, bool degenerate, double bmin, double slope,
+                    RunParams rp, N4_LDS unsigned char* smem) {
     VolState* st = V.st;
-    // the state fields of the step, in one round of loads
-    const int kind = st->s_kind, degenerate = st->degenerate, s_it = st->s_it;
-    const double bmin = st->bin_min, slope = st->slope;
     if (kind == kStepIter) {
-        if (!degenerate) wiener_table<false>(V, bmin, slope, rp, smem);
+        if (!degenerate) wiener_table<WM>(V, bmin, slope, rp, smem);
         BLOCK_FOR(t)
         if (t == 0) {
             st->mode = kModeIter;
@@ -1008,7 +994,7 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, N4_LDS unsigned 
         }
         BLOCK_END
     } else if (kind == kStepTrans) {
-        const int l = st->s_lvl;
+        const int l = s_lvl;
         BLOCK_FOR(t)
         if (l > 0) refine_lattice<false>(V, l, t);
         if (t == 0) {
@@ -1029,6 +1015,16 @@ HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, N4_LDS unsigned 
         }
         BLOCK_END
     }
+}
+
+// One block per volume: the step as the histogram kernel committed it.
+HD void body_wiener(const VolDesc* vols, RunParams rp, int bid, N4_LDS unsigned char* smem) {
+    const VolDesc& V = vols[bid];
+    const VolState* st = V.st;
+    // the state fields of the step, in one round of loads
+    const int kind = st->s_kind, degenerate = st->degenerate, s_it = st->s_it, s_lvl = st->s_lvl;
+    const double bmin = st->bin_min, slope = st->slope;
+    volume_step<kWienerLocal>(V, kind, s_lvl, s_it, degenerate != 0, bmin, slope, rp, smem);
 }
 
 // ---------------------------------------------------------- lattice update --
@@ -1052,14 +1048,58 @@ HD void scatter_numerator(const VolDesc& V, const LevelDesc& LD, const AxisEntry
     }
 }
 
-// Lattice update between the fit and the evaluation of a step, one thread
-// per control point l of the volume's current level (block `chunk` of the
-// volume covers l = chunk*kBlock ..):
-//   ITER   lres = num / (den + eps) (N4.py:236); lat += lres (N4.py:317);
-//          lresf = lres in FP32 for the evaluation
-//   TRANS  den = num (the level's denominator)
-// num is the exact sum of the numerator copies, which are cleared for the
-// next fit.
+// Lattice update of control point l (N4.py:236, 317): num = the exact sum
+// of the numerator copies (2^-30 fixed point), which are cleared for the
+// next scatter; TRANS: den = num (the level's denominator); ITER:
+// lres = num / (den + eps), lat += lres, lresf = lres (FP32 for the
+// evaluation).  AGENT_IN: the copies were filled in the same launch (or
+// team) and are read and cleared with agent-scope accesses; AGENT_OUT: den,
+// the lattice and lresf are shared with other workgroups of the launch.
+template <bool AGENT_IN, bool AGENT_OUT>
+HD void update_cp(const VolDesc& V, int l, bool trans, int level, double eps) {
+    uint64_t* p = V.numfx + l;
+    const int nc = V.num_copies;
+    // all loads first (independent), then the clears
+    uint64_t c[kMaxCopies];
+    UNROLL for (int k = 0; k < kMaxCopies; ++k) {
+        if (AGENT_IN)
+            c[k] = k < nc ? ld_agent(p + (size_t)k * V.lmax) : 0;
+        else
+            c[k] = k < nc ? p[(size_t)k * V.lmax] : 0;
+    }
+    uint64_t sum = 0;
+    UNROLL for (int k = 0; k < kMaxCopies; ++k) sum += c[k];
+    UNROLL for (int k = 0; k < kMaxCopies; ++k) {
+        if (k >= nc) continue;
+        if (AGENT_IN)
+            st_agent(p + (size_t)k * V.lmax, (uint64_t)0);
+        else
+            p[(size_t)k * V.lmax] = 0;
+    }
+    const double num = (double)(long long)sum / kNumScale;
+    double* lat = V.lat[level & 1] + l;
+    if (AGENT_OUT) {
+        if (trans) {
+            st_agent(V.den + l, num);
+        } else {
+            const double lres = num / (ld_agent(V.den + l) + eps);
+            st_agent(lat, ld_agent(lat) + lres);
+            st_agent(V.lresf + l, (float)lres);
+        }
+    } else {
+        if (trans) {
+            V.den[l] = num;
+        } else {
+            const double lres = num / (V.den[l] + eps);
+            *lat += lres;
+            V.lresf[l] = (float)lres;
+        }
+    }
+}
+
+// Lattice update between the fit and the evaluation of a step,
+// one thread per control point l of the volume's current level (block
+// `chunk` of the volume covers l = chunk*kBlock ..).
 HD void body_update(const VolDesc* vols, RunParams rp, int chunk, int vol) {
     const VolDesc& V = vols[vol];
     const VolState* st = V.st;
@@ -1069,25 +1109,7 @@ HD void body_update(const VolDesc* vols, RunParams rp, int chunk, int vol) {
     const int L = V.lev[level].L;
     BLOCK_FOR(t)
     const int l = chunk * kBlock + t;
-    if (l < L) {
-        // all loads first (independent), then the clears
-        uint64_t* p = V.numfx + l;
-        const int nc = V.num_copies;
-        uint64_t c[kMaxCopies];
-        UNROLL for (int k = 0; k < kMaxCopies; ++k) c[k] = k < nc ? p[(size_t)k * V.lmax] : 0;
-        uint64_t sum = 0;
-        UNROLL for (int k = 0; k < kMaxCopies; ++k) sum += c[k];
-        UNROLL for (int k = 0; k < kMaxCopies; ++k)
-            if (k < nc) p[(size_t)k * V.lmax] = 0;
-        const double num = (double)(long long)sum / kNumScale;
-        if (kind == kStepTrans) {
-            V.den[l] = num;
-        } else {
-            const double lres = num / (V.den[l] + rp.eps);
-            V.lat[level & 1][l] += lres;
-            V.lresf[l] = (float)lres;
-        }
-    }
+    if (l < L) update_cp<false, false>(V, l, kind == kStepTrans, level, rp.eps);
     BLOCK_END
 }
 
@@ -1803,6 +1825,38 @@ namespace n4 {
 // denominator (own tiles) and, after a barrier, moves it to den (owners).
 constexpr uint32_t kSpinLimit = 1u << 23;  // polls of a team barrier before it gives up (seconds)
 
+// Phase timer of a team's leader: device clock (100 MHz), thread 0 only,
+// accumulated into V.tphase when the caller asked for it.
+HD int threadIdx_x() {
+#ifdef N4_DEVICE_PASS
+    return (int)threadIdx.x;
+#else
+    return 0;
+#endif
+}
+
+struct PhaseTimer {
+    uint64_t* slots;
+    uint64_t last;
+    HD static uint64_t now() {
+#ifdef N4_DEVICE_PASS
+        return __builtin_amdgcn_s_memrealtime();
+#else
+        return 0;
+#endif
+    }
+    HD void mark(int phase) {
+#ifdef N4_DEVICE_PASS
+        if (!slots || threadIdx.x != 0) return;
+        const uint64_t t = now();
+        slots[phase] += t - last;
+        last = t;
+#else
+        (void)phase;
+#endif
+    }
+};
+
 HD void spin_pause() {
 #if defined(N4_DEVICE_PASS)
     __builtin_amdgcn_s_sleep(2);
@@ -1888,31 +1942,12 @@ HD void owned_range(int L, int rank, int nteam, int* o0, int* o1) {
     *o1 = (int)((int64_t)L * (rank + 1) / nteam);
 }
 
-// Owners' share of a step's lattice update, as body_update: num = the sum
-// of the numerator copies (cleared for the next scatter); TRANS: den = num;
-// ITER: lres = num/(den+eps) (N4.py:236), lattice += lres (N4.py:317),
-// lresf = lres.
+// Owners' share of a step's lattice update (update_cp with agent-scope
+// accesses throughout: the copies were filled by the team, and den, the
+// lattice and lresf are read by other members).
 HD void owner_update(const VolDesc& V, int level, bool trans, double eps, int o0, int o1) {
     BLOCK_FOR(t)
-    for (int l = o0 + t; l < o1; l += kBlock) {
-        uint64_t* p = V.numfx + l;
-        const int nc = V.num_copies;
-        uint64_t c[kMaxCopies];
-        UNROLL for (int k = 0; k < kMaxCopies; ++k) c[k] = k < nc ? ld_agent(p + (size_t)k * V.lmax) : 0;
-        uint64_t sum = 0;
-        UNROLL for (int k = 0; k < kMaxCopies; ++k) sum += c[k];
-        UNROLL for (int k = 0; k < kMaxCopies; ++k)
-            if (k < nc) st_agent(p + (size_t)k * V.lmax, (uint64_t)0);
-        const double num = (double)(long long)sum / kNumScale;
-        if (trans) {
-            st_agent(V.den + l, num);
-        } else {
-            const double lres = num / (ld_agent(V.den + l) + eps);
-            double* lat = V.lat[level & 1] + l;
-            st_agent(lat, ld_agent(lat) + lres);
-            st_agent(V.lresf + l, (float)lres);
-        }
-    }
+    for (int l = o0 + t; l < o1; l += kBlock) update_cp<true, true>(V, l, trans, level, eps);
     BLOCK_END
 }
 
@@ -2108,8 +2143,10 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
     int mode = kModeTrans, lvl = 0, it = 0;
     int pa = 0;  // first increment plane the own tiles touch (current level)
     bool need_b1 = false;
+    PhaseTimer pt{leader ? V.tphase : nullptr, PhaseTimer::now()};
     for (;;) {
         if (need_b1 && !team_barrier(V, W.nteam, gen, S.bc)) return;  // B1
+        pt.mark(kPhWait);
         reduce_partials_team(V.spart, ntiles, S.red);
         const double mn = S.red[2 * kBlock], mx = S.red[3 * kBlock];
         double cvv = 0.0;
@@ -2151,6 +2188,8 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
                 st->passes += 1;
             }
             BLOCK_END
+            pt.mark(kPhReduce);
+            if (pt.slots && threadIdx_x() == 0) pt.slots[kPhSteps] += 1;
         }
         if (kind == kStepFinal) {
             if (leader) {
@@ -2176,12 +2215,15 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
             }
             persist_tables<PREC>(V, LD, S, true);
             persist_fit<PREC>(V, LD, my, nmy, S, true, false, Remap{S.F.E, 0.0f, 0.0f, nb});
+            pt.mark(kPhTrans);
             if (!team_barrier(V, W.nteam, gen, S.bc)) return;
+            pt.mark(kPhWait);
             int o0, o1;
             owned_range(LD.L, W.rank, W.nteam, &o0, &o1);
             owner_update(V, l, true, rp.eps, o0, o1);
             persist_tables<PREC>(V, LD, S, false);
             pa = nmy > 0 ? LD.ax[0][my[0].x].b : 0;
+            pt.mark(kPhTrans);
             mode = kModeIter;
             lvl = l;
             it = 0;
@@ -2199,13 +2241,18 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
             BLOCK_SYNC();
             hist_flush(S.h, hs, nb, V.hist + (size_t)(W.rank % V.hist_copies) * nb);
         }
+        pt.mark(kPhHist);
         if (!team_barrier(V, W.nteam, gen, S.bc)) return;  // B2
+        pt.mark(kPhWait);
         if (!degenerate) {
-            wiener_table<true>(V, mn, slope, rp, S.u, S.F.E);
+            wiener_table<kWienerTeam>(V, mn, slope, rp, S.u, S.F.E);
             BLOCK_SYNC();
         }
+        pt.mark(kPhWiener);
         persist_fit<PREC>(V, LD, my, nmy, S, false, degenerate, make_remap(S.F.E, mn, slope, nb));
+        pt.mark(kPhFit);
         if (!team_barrier(V, W.nteam, gen, S.bc)) return;  // B3
+        pt.mark(kPhWait);
         if (leader) {
             BLOCK_FOR(t)
             for (int i = t; i < V.hist_copies * nb; i += kBlock) st_agent(V.hist + i, (uint64_t)0);
@@ -2216,7 +2263,9 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
             owned_range(LD.L, W.rank, W.nteam, &o0, &o1);
             owner_update(V, lvl, false, rp.eps, o0, o1);
         }
+        pt.mark(kPhUpdate);
         if (!team_barrier(V, W.nteam, gen, S.bc)) return;  // B4
+        pt.mark(kPhWait);
         if (nmy > 0) {
             const int pb = LD.ax[0][my[nmy - 1].x].b + 3;
             BLOCK_FOR(t)
@@ -2225,6 +2274,7 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
             BLOCK_END
             BLOCK_SYNC();
         }
+        pt.mark(kPhUpdate);
         for (int i = 0; i < nmy; ++i) {
             const Slab sl = my[i];
             const AxisEntry ex = LD.ax[0][sl.x];
@@ -2256,6 +2306,7 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
             BLOCK_END
             BLOCK_SYNC();
         }
+        pt.mark(kPhEval);
         it += 1;
         need_b1 = true;
     }
@@ -2355,12 +2406,14 @@ __global__ __launch_bounds__(kBlock) void k_lattice(const VolDesc* __restrict__ 
 __global__ __launch_bounds__(kBlock) void k_update(const VolDesc* __restrict__ vols, RunParams rp) {
     body_update(vols, rp, (int)blockIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(kBlock) void k_eval(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, RunParams rp) {
+// The evaluation and the reconstruction are held to 5 and 4 waves/SIMD
+// (96 and 128 VGPRs), the occupancy their memory streams need.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_eval(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_eval(vols, slabs, rp, blockIdx.x, SMEM());
 }
 
-__global__ __launch_bounds__(kBlock) void k_recon(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_recon(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level,
                                                   RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_recon(vols, slabs, level, rp, blockIdx.x, SMEM());

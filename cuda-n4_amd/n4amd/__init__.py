@@ -37,7 +37,10 @@ EXPORTS = [
     "n4_memcpy_dtoh", "n4_memcpy_dtod", "n4_memset", "n4_synchronize", "n4_profile_enable",
     "n4_profile_read", "n4_get_tuning", "n4_set_tuning", "n4_autotune_batch",
 ]
-KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice", "update", "teams"]
+KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice", "update", "teams",
+           "t_wait", "t_reduce", "t_hist", "t_wiener", "t_fit", "t_update", "t_eval", "t_trans"]
+# launch kernels (the t_* entries are phases timed inside the persistent launch)
+LAUNCH_KERNELS = KERNELS[:11]
 
 
 class Params(C.Structure):

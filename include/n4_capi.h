@@ -202,10 +202,11 @@ typedef struct n4_tuning {
     uint32_t eval_kernel;      /* field-increment kernel of the MFMA
                                   precisions: N4_EVAL_FP32 (default) or
                                   N4_EVAL_MFMA; FP32 runs use FP32 */
-    uint32_t path;             /* N4_PATH_AUTO (default): the persistent team
-                                  kernel where the batch is eligible, else the
-                                  step kernels; N4_PATH_STEPS; N4_PATH_TEAMS
-                                  (N4_ERR_UNSUPPORTED where not eligible) */
+    uint32_t path;             /* N4_PATH_AUTO (default: the step kernels,
+                                  the faster path on every workload measured);
+                                  N4_PATH_STEPS; N4_PATH_TEAMS: the persistent
+                                  team kernel (N4_ERR_UNSUPPORTED where the
+                                  batch is not eligible) */
     uint32_t team_wgs_per_cu;  /* persistent path: workgroups per compute
                                   unit (0 = as many as fit, at most 4) */
     float ms;                  /* n4_autotune_batch: wall time of one batch */
@@ -241,7 +242,18 @@ typedef enum n4_kernel_id {
     N4_K_LATTICE = 8, /* mode-x contraction + lattice update (stage calls) */
     N4_K_UPDATE = 9,  /* lattice update of a correction step */
     N4_K_TEAMS = 10,  /* persistent path: every volume's level/iteration loop */
-    N4_K_COUNT = 11
+    /* Phases of the persistent path, timed inside the launch by each team's
+     * leader workgroup (device clock; total over the volumes' leaders,
+     * launches = iteration steps).  Enabled with the N4_K_TEAMS bit. */
+    N4_K_T_WAIT = 11,   /* team barrier waits */
+    N4_K_T_REDUCE = 12, /* partial reduction and step decision */
+    N4_K_T_HIST = 13,   /* histogram of the own tiles */
+    N4_K_T_WIENER = 14, /* Wiener step (E table) */
+    N4_K_T_FIT = 15,    /* fit of the own tiles */
+    N4_K_T_UPDATE = 16, /* owned control points, increment planes */
+    N4_K_T_EVAL = 17,   /* evaluation of the own tiles */
+    N4_K_T_TRANS = 18,  /* level transitions (refinement, denominator) */
+    N4_K_COUNT = 19
 } n4_kernel_id;
 /* Time every launch of the kernels in kernel_mask (bit = n4_kernel_id) with
  * a pair of HIP events recorded on the launch stream around it (step graphs

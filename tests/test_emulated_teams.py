@@ -84,3 +84,4 @@ def test_teams_refused_where_not_eligible(emu):
     finally:
         emu.set_tuning(reset=True)
     emu.correct(im, m, **kw)
+
