@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -104,6 +105,7 @@ struct n4_handle {
     double prof_ms[N4_K_COUNT] = {0};
     uint64_t prof_n[N4_K_COUNT] = {0};
     n4_tuning tune{};  // launch schedule; 0 fields = defaults
+    void* scratch = nullptr;  // the device's allocation-free scratch array (device address)
 #ifndef N4_EMULATE
     // Captured chunks of iteration steps, one per group stream, reused while
     // their launch arguments (the key) stay the same.
@@ -114,6 +116,42 @@ struct n4_handle {
     hipStream_t cap_stream = nullptr;
 #endif
 };
+
+// ------------------------------------------------- allocation-free scratch --
+// n4_correct_noalloc keeps its plan and working buffers in this file-scope
+// array (each device that runs the library holds its own copy until the
+// process ends) instead of the handle's arena.  Host code never uses the
+// array's name as an address: the device address is resolved once per
+// device (hipGetSymbolAddress, at n4_create with the device current) and
+// kept; calls that overlap on one device race on it.
+static const size_t kScratchBytes = (size_t)64 << 20;
+#ifndef N4_EMULATE
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+#else
+alignas(256) static unsigned char scratch[kScratchBytes];
+#endif
+static const int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address (resolved on first use).
+static n4_status resolve_scratch(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(N4_ERR_INVALID_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+#ifndef N4_EMULATE
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(N4_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+#else
+        g_scratch_ptr[device] = scratch;
+#endif
+    }
+    *out = g_scratch_ptr[device];
+    return N4_OK;
+}
 
 // Early exit: each histogram pass writes the volume's progress word (level,
 // passes run, done) straight into pinned host memory.  The host keeps at
@@ -249,6 +287,7 @@ struct Plan {
     size_t off_work = 0;     // PWork[n_work] in the upload region
     int n_work = 0;
     size_t lds_persist = 0;  // dynamic LDS of the persistent kernel (largest volume)
+    unsigned char* base = nullptr;  // device arena the plan lives in
 };
 
 static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
@@ -359,9 +398,12 @@ static int rows_for(int rows, int cols, int target) {
 // holds the PWork list (consecutive ranks per volume, each rank a
 // contiguous share of the volume's tiles), a barrier slot per volume and
 // histogram copies sized to the team.
+// noalloc: the plan lives in the device's file-scope scratch array (no
+// allocation; N4_ERR_NO_MEMORY beyond its size) and carries no progress
+// words (pinned host memory).
 static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4_params* p, Plan* P,
                             bool need_tables_full = true, bool tiled = false, int tile_vox = kStepTileVox,
-                            const std::vector<int>* teams = nullptr) {
+                            const std::vector<int>* teams = nullptr, bool noalloc = false) {
     const int nv = (int)in.size();
     const int L = (int)p->number_of_fitting_levels;
     P->nvol = nv;
@@ -459,9 +501,19 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     }
     P->total_bytes = lay.take(0);
 
-    n4_status s = ensure_arena(h, P->total_bytes);
-    if (s != N4_OK) return s;
-    unsigned char* base = (unsigned char*)h->arena;
+    unsigned char* base = nullptr;
+    if (noalloc) {
+        if (!h->scratch) return fail(N4_ERR_INTERNAL, "allocation-free scratch of device %d not resolved", h->device);
+        if (P->total_bytes > kScratchBytes)
+            return fail(N4_ERR_NO_MEMORY, "the request needs %zu bytes of device scratch; the allocation-free entry "
+                        "point has %zu (use n4_correct)", P->total_bytes, (size_t)kScratchBytes);
+        base = (unsigned char*)h->scratch;
+    } else {
+        n4_status s = ensure_arena(h, P->total_bytes);
+        if (s != N4_OK) return s;
+        base = (unsigned char*)h->arena;
+    }
+    P->base = base;
 
     // ---- host image of the upload region
     h->staging.assign(P->upload_bytes, 0);
@@ -524,7 +576,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.mF2 = magic_div((uint32_t)in[v].F[2]);
         D.etab = (double*)(base + o.etab);
         D.st = (VolState*)(base + P->off_state) + v;
-        D.progress = (h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
+        D.progress = (!noalloc && h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
         D.nlevels = L;
         D.part_stride = (int32_t)I.part_stride;
         for (int l = 0; l < L; ++l) {
@@ -600,19 +652,19 @@ static n4_status check_lds(const Plan& P, bool grid_kernels, bool recon) {
 }
 
 static inline VolDesc* dev_desc(n4_handle* h, const Plan& P) {
-    return (VolDesc*)((unsigned char*)h->arena + P.off_desc);
+    return (VolDesc*)(P.base + P.off_desc);
 }
 static inline Slab* dev_wslab(n4_handle* h, const Plan& P) {
-    return (Slab*)((unsigned char*)h->arena + P.off_wslab);
+    return (Slab*)(P.base + P.off_wslab);
 }
 static inline Slab* dev_fslab(n4_handle* h, const Plan& P) {
-    return (Slab*)((unsigned char*)h->arena + P.off_fslab);
+    return (Slab*)(P.base + P.off_fslab);
 }
 static inline Slab* dev_pslab(n4_handle* h, const Plan& P) {
-    return (Slab*)((unsigned char*)h->arena + P.off_pslab);
+    return (Slab*)(P.base + P.off_pslab);
 }
 static inline VolState* dev_state(n4_handle* h, const Plan& P) {
-    return (VolState*)((unsigned char*)h->arena + P.off_state);
+    return (VolState*)(P.base + P.off_state);
 }
 
 // --------------------------------------------------------------- checks --
@@ -929,9 +981,14 @@ static bool plan_teams(const std::vector<VolIn>& in, const n4_params* p, const S
         LAUNCH(expr);                      \
     } while (0)
 
+// noalloc (n4_correct_noalloc): the plan lives in the device's scratch
+// array; one group on the caller's stream, every step enqueued up front
+// (no progress polling, so no pinned host words), direct launches (no graph
+// instantiation), no events, no kernel timing.
 static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, const n4_params* p, n4_report* reports,
-                           void* stream) {
+                           void* stream, bool noalloc = false) {
     g_err.clear();
+    const auto t_host0 = std::chrono::steady_clock::now();
     n4_status s = check_params(p);
     if (s != N4_OK) return s;
     for (const VolIn& v : in_user) {
@@ -943,13 +1000,20 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     const Schedule sch = resolve_schedule(h, nv);
     std::vector<int> teams_user;
     bool teams_path = false;
-    if (sch.path != (int)N4_PATH_STEPS) {
+    if (sch.path != (int)N4_PATH_STEPS && !noalloc) {
         std::string why;
         teams_path = plan_teams(in_user, p, sch, &teams_user, &why);
         if (!teams_path && sch.path == (int)N4_PATH_TEAMS)
             return fail(N4_ERR_UNSUPPORTED, "persistent team path not available for this batch: %s", why.c_str());
     }
-    const int G = teams_path ? 1 : sch.groups;
+    const int G = (teams_path || noalloc) ? 1 : sch.groups;
+    // kernel timing creates events: off for the allocation-free path
+    struct ProfMaskGuard {
+        n4_handle* h;
+        uint32_t saved;
+        ~ProfMaskGuard() { h->prof_mask = saved; }
+    } prof_guard{h, h->prof_mask};
+    if (noalloc) h->prof_mask = 0;
     // order volumes by group: largest first (working voxels), each onto the
     // group with the least work so far
     std::vector<int> order(nv), gid(nv, 0);
@@ -984,9 +1048,11 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
 #ifndef N4_EMULATE
     N4_CHECK_HIP(hipSetDevice(h->device));
 #endif
-    s = ensure_flags(h, nv);
-    if (s != N4_OK) return s;
-    for (int v = 0; v < nv; ++v) h->pinned_flags[v] = 0;
+    if (!noalloc) {
+        s = ensure_flags(h, nv);
+        if (s != N4_OK) return s;
+        for (int v = 0; v < nv; ++v) h->pinned_flags[v] = 0;
+    }
 #ifndef N4_EMULATE
     for (int g = 1; g < G; ++g)
         if (!h->gstream[g]) N4_CHECK_HIP(hipStreamCreateWithFlags(&h->gstream[g], hipStreamNonBlocking));
@@ -997,16 +1063,18 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             if (b) (void)hipEventDestroy(b);
         }
     } ev;
-    N4_CHECK_HIP(hipEventCreate(&ev.a));
-    N4_CHECK_HIP(hipEventCreate(&ev.b));
-    N4_CHECK_HIP(hipEventRecord(ev.a, (hipStream_t)stream));
+    if (!noalloc) {
+        N4_CHECK_HIP(hipEventCreate(&ev.a));
+        N4_CHECK_HIP(hipEventCreate(&ev.b));
+        N4_CHECK_HIP(hipEventRecord(ev.a, (hipStream_t)stream));
+    }
 #endif
     Plan P;
-    s = build_plan(h, in, p, &P, true, true, sch.tile_vox, teams_path ? &teams : nullptr);
+    s = build_plan(h, in, p, &P, true, true, sch.tile_vox, teams_path ? &teams : nullptr, noalloc);
     if (s != N4_OK) return s;
     s = check_lds(P, !teams_path, true);
     if (s != N4_OK) return s;
-    H2D(h->arena, h->staging.data(), P.upload_bytes, stream);
+    H2D(P.base, h->staging.data(), P.upload_bytes, stream);
     VolDesc* dd = dev_desc(h, P);
     Slab* wsl = dev_wslab(h, P);
     Slab* fsl = dev_fslab(h, P);
@@ -1053,7 +1121,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     // volume's end are skips.  Graphs are not used while kernels are timed
     // with events.
 #ifndef N4_EMULATE
-    const bool use_graph = sch.graph_steps > 0 && h->prof_mask == 0;
+    const bool use_graph = sch.graph_steps > 0 && h->prof_mask == 0 && !noalloc;
 #else
     const bool use_graph = false;
 #endif
@@ -1092,7 +1160,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     if (teams_path) {
         // one launch runs every volume's level/iteration loop
         void* gs = stream;
-        GLAUNCH(N4_K_TEAMS, launch_persist(dd, wsl, (const PWork*)((unsigned char*)h->arena + P.off_work), P.n_work,
+        GLAUNCH(N4_K_TEAMS, launch_persist(dd, wsl, (const PWork*)(P.base + P.off_work), P.n_work,
                                            rp, P.lds_persist, gs));
         GLAUNCH(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, gs));
         for (int v = 0; v < nv; ++v)
@@ -1111,7 +1179,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             if (R.finished) continue;
             void* gs = R.stream;
             bool end = R.it >= max_steps;
-            if (R.it > 0 && !end) {
+            if (R.it > 0 && !end && !noalloc) {
                 bool ready = false, all_done = false;
                 s = check_progress(h, R.v0, R.v1, 0, R.it, depth, gs, (idle_rounds & 63) == 63, &ready,
                                    &all_done);
@@ -1196,10 +1264,17 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
         }
     float ms = 0.0f;
 #ifndef N4_EMULATE
-    N4_CHECK_HIP(hipEventRecord(ev.b, (hipStream_t)stream));
-    N4_CHECK_HIP(hipEventSynchronize(ev.b));
-    N4_CHECK_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+    if (!noalloc) {
+        N4_CHECK_HIP(hipEventRecord(ev.b, (hipStream_t)stream));
+        N4_CHECK_HIP(hipEventSynchronize(ev.b));
+        N4_CHECK_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+    } else {
+        SYNC(stream);
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    }
     N4_CHECK_HIP(hipGetLastError());
+#else
+    (void)t_host0;
 #endif
     prof_collect(h);
     if (!tph.empty()) {
@@ -1310,6 +1385,14 @@ n4_status n4_create(n4_handle** out, int device) {
 #endif
     n4_handle* h = new n4_handle();
     h->device = device;
+    {
+        // the device's allocation-free scratch, resolved with the device current
+        const n4_status ss = resolve_scratch(device, &h->scratch);
+        if (ss != N4_OK) {
+            delete h;
+            return ss;
+        }
+    }
 #ifndef N4_EMULATE
     // HIP maps streams onto the process's few hardware queues in creation
     // order: the group streams are created first, in group order, so that
@@ -1365,6 +1448,16 @@ n4_status n4_correct(n4_handle* h, const float* d_im, const void* d_mask, uint32
     VolIn v{d_im, d_mask, (int)mask_kind, {(int)fx, (int)fy, (int)fz}, (int)p->shrink_factor,
             d_im_normalized, d_bias_field, d_log_bias_field, d_lattice};
     return run_batch(h, std::vector<VolIn>{v}, p, report, stream);
+}
+
+n4_status n4_correct_noalloc(n4_handle* h, const float* d_im, const void* d_mask, uint32_t mask_kind, uint32_t fx,
+                             uint32_t fy, uint32_t fz, const n4_params* p, float* d_im_normalized, float* d_lattice,
+                             float* d_bias_field, float* d_log_bias_field, n4_report* report, void* stream) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+    if (!p) return fail(N4_ERR_INVALID_ARG, "params is NULL");
+    VolIn v{d_im, d_mask, (int)mask_kind, {(int)fx, (int)fy, (int)fz}, (int)p->shrink_factor,
+            d_im_normalized, d_bias_field, d_log_bias_field, d_lattice};
+    return run_batch(h, std::vector<VolIn>{v}, p, report, stream, true);
 }
 
 n4_status n4_correct_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, const n4_params* p,
@@ -1529,7 +1622,7 @@ static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const f
         D.wmask = const_cast<uint8_t*>(m);
     }
     memcpy(h->staging.data() + P->off_desc, &D, sizeof D);
-    H2D(h->arena, h->staging.data(), P->upload_bytes, nullptr);
+    H2D(P->base, h->staging.data(), P->upload_bytes, nullptr);
     if (hist) {
         const size_t nw = (size_t)nslab * kChunkVox;
         D2D(D.logU, x, sizeof(float) * n, nullptr);
@@ -1631,7 +1724,7 @@ n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_
     s = check_lds(P, true, false);
     if (s != N4_OK) return s;
     VolDesc& D = P.desc[0];
-    H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    H2D(P.base, h->staging.data(), P.upload_bytes, nullptr);
     // the fit kernel takes 16-byte loads of aligned voxel quads: it reads
     // the plan's own padded copies of the field and the mask
     const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
@@ -1670,7 +1763,7 @@ n4_status n4_evaluate_bspline3d(n4_handle* h, const float* d_lattice, const uint
     VolDesc& D = P.desc[0];
     D.latf = const_cast<float*>(d_lattice);
     memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
-    H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    H2D(P.base, h->staging.data(), P.upload_bytes, nullptr);
     LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, nullptr));
     SYNC(nullptr);
     return N4_OK;
@@ -1718,7 +1811,7 @@ n4_status n4_upsample_lattice3d(n4_handle* h, const float* d_lattice, const uint
     Plan P;
     s = build_plan(h, std::vector<VolIn>{v}, &p, &P, false);
     if (s != N4_OK) return s;
-    H2D(h->arena, h->staging.data(), P.upload_bytes, nullptr);
+    H2D(P.base, h->staging.data(), P.upload_bytes, nullptr);
     const VolDesc& D = P.desc[0];
     const size_t L0 = (size_t)D.lev[0].L, L1 = (size_t)D.lev[1].L;
     std::vector<float> lf(L0);

@@ -30,7 +30,7 @@ FLAG_FIXED_ITERATIONS = 0x1
 # Names every build of the library must export (include/n4_capi.h).
 EXPORTS = [
     "n4_version", "n4_status_string", "n4_last_error", "n4_params_default", "n4_final_ncpt",
-    "n4_create", "n4_destroy", "n4_correct", "n4_correct_host", "n4_correct_batch",
+    "n4_create", "n4_destroy", "n4_correct", "n4_correct_noalloc", "n4_correct_host", "n4_correct_batch",
     "n4_histogram", "n4_sharpen_image", "n4_fit_bspline3d", "n4_evaluate_bspline3d",
     "n4_convergence_measurement", "n4_upsample_lattice3d", "n4_reduce_sum", "n4_reduce_max",
     "n4_reduce_min", "n4_device_count", "n4_malloc", "n4_free", "n4_memcpy_htod",
@@ -129,6 +129,8 @@ def _declare(L):
         "n4_destroy": ([vp], C.c_int),
         "n4_correct": ([vp, vp, vp, u32, u32, u32, u32, C.POINTER(Params), vp, vp, vp, vp,
                         C.POINTER(Report), vp], C.c_int),
+        "n4_correct_noalloc": ([vp, vp, vp, u32, u32, u32, u32, C.POINTER(Params), vp, vp, vp, vp,
+                                C.POINTER(Report), vp], C.c_int),
         "n4_correct_host": ([vp, vp, vp, u32, u32, u32, C.POINTER(Params), vp, vp, vp,
                              C.POINTER(Report)], C.c_int),
         "n4_correct_batch": ([vp, C.POINTER(Volume), u32, C.POINTER(Params), C.POINTER(Report), vp], C.c_int),
@@ -339,9 +341,10 @@ class Library:
 
     # ------------------------------------------------------------ N4 ----
     def correct(self, im, mask=None, params: Params | None = None, want_bias=False, want_log_bias=True,
-                mask_kind=None, **kw):
+                mask_kind=None, noalloc=False, **kw):
         """Whole correction of one volume.  Returns dict with normalized,
-        log_bias (optional), bias (optional), lattice, report."""
+        log_bias (optional), bias (optional), lattice, report.  noalloc: the
+        allocation-free entry point (n4_correct_noalloc)."""
         p = params if params is not None else default_params(**kw)
         im = np.ascontiguousarray(im, dtype=np.float32)
         dims = im.shape
@@ -359,9 +362,10 @@ class Library:
         d_bias = self.empty(4 * n) if want_bias else None
         d_lb = self.empty(4 * n) if want_log_bias else None
         rep = Report()
-        self._check(self.L.n4_correct(self.h, d_im.ptr, d_mask.ptr if d_mask else None, mk, dims[0], dims[1],
-                                      dims[2], C.byref(p), d_out.ptr, d_lat.ptr, d_bias.ptr if d_bias else None,
-                                      d_lb.ptr if d_lb else None, C.byref(rep), None))
+        entry = self.L.n4_correct_noalloc if noalloc else self.L.n4_correct
+        self._check(entry(self.h, d_im.ptr, d_mask.ptr if d_mask else None, mk, dims[0], dims[1], dims[2],
+                          C.byref(p), d_out.ptr, d_lat.ptr, d_bias.ptr if d_bias else None,
+                          d_lb.ptr if d_lb else None, C.byref(rep), None))
         out = {
             "normalized": d_out.download(np.float32, dims),
             "lattice": d_lat.download(np.float32, nc),

@@ -120,6 +120,24 @@ n4_status n4_correct(n4_handle *h, const float *d_im, const void *d_mask,
                      float *d_lattice, float *d_bias_field,
                      float *d_log_bias_field, n4_report *report, void *stream);
 
+/* n4_correct without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as n4_correct; the plan and working
+ * buffers live in a file-scope device array of 64 MiB per device (held
+ * until the process ends) instead of the handle's arena, so the working
+ * grid is limited (about 6.5 M working voxels at the default parameters);
+ * larger requests fail with N4_ERR_NO_MEMORY.  It runs the step kernels on
+ * `stream` with every step enqueued up front (no progress words in pinned
+ * memory, no events, no graph instantiation; kernel timing is ignored) and
+ * returns after the work completes.  Calls that overlap on one device race
+ * on the shared array.  Call on the handle's device. */
+n4_status n4_correct_noalloc(n4_handle *h, const float *d_im, const void *d_mask,
+                             uint32_t mask_kind, uint32_t fx, uint32_t fy, uint32_t fz,
+                             const n4_params *p, float *d_im_normalized,
+                             float *d_lattice, float *d_bias_field,
+                             float *d_log_bias_field, n4_report *report, void *stream);
+
 /* Replaces testN4 (cudaN4.h:367-387, cudaN4.cu:1751-1847): host in/out. */
 n4_status n4_correct_host(n4_handle *h, const float *h_im, const float *h_mask,
                           uint32_t fx, uint32_t fy, uint32_t fz,
