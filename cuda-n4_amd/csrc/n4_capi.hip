@@ -16,6 +16,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -106,6 +108,22 @@ struct n4_handle {
     uint64_t prof_n[N4_K_COUNT] = {0};
     n4_tuning tune{};  // launch schedule; 0 fields = defaults
     void* scratch = nullptr;  // the device's allocation-free scratch array (device address)
+    // Run records of the last correction call (n4_last_cv_trace,
+    // n4_last_level_lattice): device addresses in the arena (or the
+    // scratch array) and the iterations per level, per volume in the
+    // caller's order.  Any later plan reuses that memory and clears `valid`.
+    struct LastRun {
+        bool valid = false;
+        bool in_scratch = false;     // allocation-free call: valid while the scratch generation holds
+        uint64_t scratch_gen = 0;
+        int nlevels = 0;
+        std::vector<int> max_iter;   // per level
+        std::vector<const double*> cvtrace;
+        std::vector<int> trace_cap;
+        std::vector<const float*> lvlat;
+        std::vector<std::array<uint32_t, kMaxLevels>> iters;
+        std::vector<std::array<int32_t, 3 * kMaxLevels>> ncpt;
+    } last;
 #ifndef N4_EMULATE
     // Captured chunks of iteration steps, one per group stream, reused while
     // their launch arguments (the key) stay the same.
@@ -133,6 +151,13 @@ alignas(256) static unsigned char scratch[kScratchBytes];
 static const int kMaxDevices = 64;
 static std::mutex g_scratch_mu;
 static void* g_scratch_ptr[kMaxDevices] = {};
+// allocation-free calls started per device (a run record in the scratch is
+// valid while this count is unchanged)
+static std::atomic<uint64_t> g_scratch_gen[kMaxDevices];
+
+static uint64_t scratch_generation(int device) {
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_gen[device].load() : 0;
+}
 
 // The current device's scratch address (resolved on first use).
 static n4_status resolve_scratch(int device, void** out) {
@@ -314,6 +339,9 @@ static uint64_t magic_div(uint32_t d) { return ((uint64_t)1 << kDivShift) / d + 
 static const int kPrepTileVox = 512;
 // Default working voxels per tile of the step kernels.
 static const int kStepTileVox = 8192;
+// Convergence values recorded per volume and call (iterations after the
+// first kTraceCap of a call are not recorded).
+static const int kTraceCap = 4096;
 static const int kReconTileVox = 8192;
 // Default iteration steps per captured graph, and the largest chunk.
 static const int kGraphSteps = 8;
@@ -406,6 +434,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
                             const std::vector<int>* teams = nullptr, bool noalloc = false) {
     const int nv = (int)in.size();
     const int L = (int)p->number_of_fitting_levels;
+    h->last.valid = false;  // the plan reuses the memory of the last run's records
     P->nvol = nv;
     P->nlevels = L;
     P->rp = make_run_params(p);
@@ -470,13 +499,19 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
 
     // per-volume working buffers
     struct Offs {
-        size_t imlog, logU, wmask, lat0, lat1, den, latf, lresf, part, numfx, hist, spart, etab, ppart, team;
+        size_t imlog, logU, wmask, wconf, lat0, lat1, den, latf, lresf, part, numfx, hist, spart, etab, ppart, team,
+            trace, lvlat;
     };
     // histogram copies: per tile (step kernels) or per team member (teams)
     auto hcopies = [&](int v) {
         if (teams) return std::max(1, std::min(kHistCopies, ((*teams)[v] + 7) / 8));
         return hist_copies(vi[v].W[0] * vi[v].wtiles_x, tiled);
     };
+    int trace_cap = 0;
+    for (int l = 0; l < L; ++l) trace_cap += (int)std::min<uint32_t>(p->max_iterations[l], (uint32_t)kTraceCap);
+    trace_cap = std::min(trace_cap, kTraceCap);
+    // confidence weights of the working grid: F32 masks under N4_FLAG_WEIGHTED_MASK
+    auto weighted = [&](int v) { return (p->flags & N4_FLAG_WEIGHTED_MASK) && in[v].mask_kind == N4_MASK_F32; };
     std::vector<Offs> off(nv);
     const int nb = P->rp.nbins;
     for (int v = 0; v < nv; ++v) {
@@ -486,6 +521,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.imlog = lay.take(sizeof(float) * nw);
         o.logU = lay.take(sizeof(float) * (nw + kQuadPad));
         o.wmask = lay.take(nw + kQuadPad);
+        o.wconf = weighted(v) ? lay.take(sizeof(float) * nw) : 0;
         o.lat0 = lay.take(sizeof(double) * I.Lmax);
         o.lat1 = lay.take(sizeof(double) * I.Lmax);
         o.den = lay.take(sizeof(double) * I.Lmax);
@@ -498,6 +534,11 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         o.ppart = lay.take(sizeof(double) * 4 * I.W[0] * I.prep_tiles_x);
         o.etab = lay.take(sizeof(double) * nb);
         o.team = teams ? lay.take(64 + sizeof(uint64_t) * kPhases) : 0;
+        // run records of a correction (tiled plans): CV trace, level lattices
+        o.trace = tiled ? lay.take(sizeof(double) * trace_cap) : 0;
+        size_t lsum = 0;
+        for (int l = 0; l < L; ++l) lsum += (size_t)I.C[l][0] * I.C[l][1] * I.C[l][2];
+        o.lvlat = tiled ? lay.take(sizeof(float) * lsum) : 0;
     }
     P->total_bytes = lay.take(0);
 
@@ -508,6 +549,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             return fail(N4_ERR_NO_MEMORY, "the request needs %zu bytes of device scratch; the allocation-free entry "
                         "point has %zu (use n4_correct)", P->total_bytes, (size_t)kScratchBytes);
         base = (unsigned char*)h->scratch;
+        if (h->device >= 0 && h->device < kMaxDevices) g_scratch_gen[h->device]++;
     } else {
         n4_status s = ensure_arena(h, P->total_bytes);
         if (s != N4_OK) return s;
@@ -552,6 +594,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.imlog = (float*)(base + o.imlog);
         D.logU = (float*)(base + o.logU);
         D.wmask = (uint8_t*)(base + o.wmask);
+        D.wconf = weighted(v) ? (float*)(base + o.wconf) : nullptr;
         D.lat[0] = (double*)(base + o.lat0);
         D.lat[1] = (double*)(base + o.lat1);
         D.den = (double*)(base + o.den);
@@ -577,6 +620,9 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         D.etab = (double*)(base + o.etab);
         D.st = (VolState*)(base + P->off_state) + v;
         D.progress = (!noalloc && h->dev_flags && v < h->flag_cap) ? h->dev_flags + v : nullptr;
+        D.cvtrace = tiled ? (double*)(base + o.trace) : nullptr;
+        D.trace_cap = tiled ? trace_cap : 0;
+        D.lvlat = tiled ? (float*)(base + o.lvlat) : nullptr;
         D.nlevels = L;
         D.part_stride = (int32_t)I.part_stride;
         for (int l = 0; l < L; ++l) {
@@ -687,6 +733,8 @@ static n4_status check_params(const n4_params* p) {
     if (p->shrink_factor < 1 || p->shrink_factor > 64) return fail(N4_ERR_INVALID_ARG, "shrink_factor must be in [1, 64]");
     if (p->precision != N4_PREC_FP32 && p->precision != N4_PREC_BF16 && p->precision != N4_PREC_SPLIT_BF16)
         return fail(N4_ERR_INVALID_ARG, "precision %u is not an n4_precision value", p->precision);
+    if (p->flags & ~(N4_FLAG_FIXED_ITERATIONS | N4_FLAG_WEIGHTED_MASK))
+        return fail(N4_ERR_INVALID_ARG, "flags 0x%x: unknown bits", p->flags);
 #ifdef N4_EMULATE
     if (p->precision != N4_PREC_FP32)
         return fail(N4_ERR_UNSUPPORTED, "precision %u: the host emulation build runs the FP32 kernels only",
@@ -1290,6 +1338,29 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     for (int v = 0; v < (int)tctr.size() / 2; ++v)
         if (tctr[2 * v + 1])
             return fail(N4_ERR_INTERNAL, "persistent path: a team barrier of volume %d timed out", order[v]);
+    {
+        n4_handle::LastRun& R = h->last;
+        R.nlevels = P.nlevels;
+        R.max_iter = P.max_iter;
+        R.cvtrace.assign(nv, nullptr);
+        R.trace_cap.assign(nv, 0);
+        R.lvlat.assign(nv, nullptr);
+        R.iters.assign(nv, {});
+        R.ncpt.assign(nv, {});
+        for (int k = 0; k < nv; ++k) {
+            const int v = order[k];
+            R.cvtrace[v] = P.desc[k].cvtrace;
+            R.trace_cap[v] = P.desc[k].trace_cap;
+            R.lvlat[v] = P.desc[k].lvlat;
+            for (int l = 0; l < P.nlevels; ++l) {
+                R.iters[v][l] = (uint32_t)st[k].iters[l];
+                for (int d = 0; d < 3; ++d) R.ncpt[v][3 * l + d] = P.desc[k].lev[l].C[d];
+            }
+        }
+        R.in_scratch = noalloc;
+        R.scratch_gen = noalloc ? scratch_generation(h->device) : 0;
+        R.valid = true;
+    }
     if (reports)
         for (int k = 0; k < nv; ++k) {
             const int v = order[k];
@@ -1458,6 +1529,76 @@ n4_status n4_correct_noalloc(n4_handle* h, const float* d_im, const void* d_mask
     VolIn v{d_im, d_mask, (int)mask_kind, {(int)fx, (int)fy, (int)fz}, (int)p->shrink_factor,
             d_im_normalized, d_bias_field, d_log_bias_field, d_lattice};
     return run_batch(h, std::vector<VolIn>{v}, p, report, stream, true);
+}
+
+// The run record of volume `volume` of the last correction call, checked.
+static n4_status last_run(n4_handle* h, uint32_t volume, const n4_handle::LastRun** out) {
+    if (!h) return fail(N4_ERR_INVALID_ARG, "handle is NULL");
+    const n4_handle::LastRun& R = h->last;
+    if (!R.valid)
+        return fail(N4_ERR_INVALID_ARG, "no run record: no correction ran on this handle since its last plan "
+                    "(stage calls reuse the same memory)");
+    if (R.in_scratch && R.scratch_gen != scratch_generation(h->device))
+        return fail(N4_ERR_INVALID_ARG, "no run record: another allocation-free call ran on device %d since",
+                    h->device);
+    if (volume >= R.cvtrace.size())
+        return fail(N4_ERR_INVALID_ARG, "volume %u: the last call had %zu", volume, R.cvtrace.size());
+    *out = &R;
+    return N4_OK;
+}
+
+n4_status n4_last_cv_trace(n4_handle* h, uint32_t volume, double* out, uint32_t capacity, uint32_t* count) {
+    const n4_handle::LastRun* R = nullptr;
+    n4_status s = last_run(h, volume, &R);
+    if (s != N4_OK) return s;
+    if (!count) return fail(N4_ERR_INVALID_ARG, "count is NULL");
+    if (capacity > 0 && !out) return fail(N4_ERR_INVALID_ARG, "out is NULL");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+#endif
+    // level l's values sit at the sum of the earlier levels' maxima
+    uint32_t n = 0;
+    int off = 0;
+    for (int l = 0; l < R->nlevels; ++l) {
+        const uint32_t it = R->iters[volume][l];
+        const int avail = std::max(0, std::min<int>((int)it, R->trace_cap[volume] - off));
+        const uint32_t take = std::min<uint32_t>((uint32_t)avail, capacity > n ? capacity - n : 0u);
+        if (take > 0) {
+#ifndef N4_EMULATE
+            N4_CHECK_HIP(hipMemcpy(out + n, R->cvtrace[volume] + off, sizeof(double) * take, hipMemcpyDeviceToHost));
+#else
+            memcpy(out + n, R->cvtrace[volume] + off, sizeof(double) * take);
+#endif
+        }
+        n += (uint32_t)avail;
+        off += R->max_iter[l];
+    }
+    *count = n;
+    return N4_OK;
+}
+
+n4_status n4_last_level_lattice(n4_handle* h, uint32_t volume, uint32_t level, float* out, uint64_t capacity) {
+    const n4_handle::LastRun* R = nullptr;
+    n4_status s = last_run(h, volume, &R);
+    if (s != N4_OK) return s;
+    if (level >= (uint32_t)R->nlevels)
+        return fail(N4_ERR_INVALID_ARG, "level %u: the last call had %d", level, R->nlevels);
+    size_t off = 0, L = 0;
+    for (uint32_t l = 0; l <= level; ++l) {
+        const int32_t* c = &R->ncpt[volume][3 * l];
+        L = (size_t)c[0] * c[1] * c[2];
+        if (l < level) off += L;
+    }
+    if (capacity < L) return fail(N4_ERR_INVALID_ARG, "level %u lattice has %zu values, capacity %llu", level, L,
+                                  (unsigned long long)capacity);
+    if (!out) return fail(N4_ERR_INVALID_ARG, "out is NULL");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+    N4_CHECK_HIP(hipMemcpy(out, R->lvlat[volume] + off, sizeof(float) * L, hipMemcpyDeviceToHost));
+#else
+    memcpy(out, R->lvlat[volume] + off, sizeof(float) * L);
+#endif
+    return N4_OK;
 }
 
 n4_status n4_correct_batch(n4_handle* h, const n4_volume* vols, uint32_t nvol, const n4_params* p,
@@ -1794,6 +1935,17 @@ n4_status n4_convergence_measurement(n4_handle* h, const float* d_f1, const floa
         cvv = sqrt(var) / (1.0 + mu);
     }
     *out = cvv;
+    return N4_OK;
+}
+
+n4_status n4_mask_to_u8(n4_handle* h, const float* d_mask, uint64_t n, uint8_t* d_out) {
+    g_err.clear();
+    if (!h || !d_mask || !d_out || n == 0) return fail(N4_ERR_INVALID_ARG, "NULL argument or empty input");
+#ifndef N4_EMULATE
+    N4_CHECK_HIP(hipSetDevice(h->device));
+#endif
+    LAUNCH(launch_mask_u8(d_mask, d_out, n, nullptr));
+    SYNC(nullptr);
     return N4_OK;
 }
 

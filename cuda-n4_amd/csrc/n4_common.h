@@ -3,7 +3,7 @@
 // Device memory layout (per volume, all in one arena owned by the handle):
 //   full resolution (caller's buffers): im f32, mask f32|u8, outputs f32
 //   working grid W = ceil(F/s) per axis, C order [x][y][z]:
-//     imlog f32, logU f32, wmask u8
+//     imlog f32, logU f32, wmask u8, wconf f32 (weighted masks only)
 //   lattice: lat[2] f64 (ping-pong across levels), den f64, latf f32,
 //            lresf f32 (largest level size)
 //   per-slab partials (one slab = one working x index):
@@ -92,6 +92,7 @@ struct VolDesc {
     float* imlog;
     float* logU;
     uint8_t* wmask;
+    float* wconf;        // confidence weight per working voxel (weighted masks; else null)
     // lattice
     double* lat[2];
     double* den;
@@ -105,6 +106,9 @@ struct VolDesc {
     double* etab;
     VolState* st;
     uint64_t* progress;        // progress word in pinned host memory (see progress_word)
+    double* cvtrace;           // convergence value of every iteration, level-major (null: none)
+    int32_t trace_cap;         // entries of cvtrace (later iterations are not recorded)
+    float* lvlat;              // each level's lattice at the level's end, level-major (null: none)
     LevelDesc lev[kMaxLevels];
     const AxisEntry* fax[3];   // full-res axis tables for the final lattice
     int32_t nlevels;

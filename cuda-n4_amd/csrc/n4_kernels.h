@@ -84,5 +84,8 @@ int launch_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, in
 enum FlatOp : int32_t { kFlatMinMax = 0, kFlatSum = 1, kFlatCV = 2, kFlatRemap = 3 };
 int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const float* a, const float* b,
                 float* out, RunParams rp, void* stream);
+// out[i] = in[i] != 0 (a float mask as the uint8 mask of the stage calls);
+// vector loads where `in` is 16-byte and `out` 4-byte aligned.
+int launch_mask_u8(const float* in, uint8_t* out, uint64_t n, void* stream);
 
 }  // namespace n4

@@ -35,6 +35,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 #if defined(__HIP_DEVICE_COMPILE__)
 #define N4_DEVICE_PASS 1
 #endif
@@ -312,15 +314,43 @@ HD int rows_per_chunk(int Y, int Z) {
 // s-float runs).  Writes imlog, logU (= imlog), wmask and the slab partial
 // (masked full voxels, foreground working voxels, min, max of imlog).
 // S > 0: shrink factor known at compile time (unrolled); S == 0: any.
-template <int S, int MK>
-HD bool prep_mask(const VolDesc& V, size_t fv) {
+// MK: the mask's kind (n4_mask_kind F32 0, U8 1, none 2) or 3, an F32
+// confidence mask (N4_FLAG_WEIGHTED_MASK): foreground where the weight is
+// > 0 and finite, and the block's mean foreground weight goes to wconf.
+template <int MK>
+using PrepMaskT = typename std::conditional<MK == 3, float, bool>::type;
+
+template <int MK>
+HD PrepMaskT<MK> prep_mask(const VolDesc& V, size_t fv) {
+    if constexpr (MK == 3) return gp(reinterpret_cast<const float*>(V.mask))[fv];
     if (MK == 0) return gp(reinterpret_cast<const float*>(V.mask))[fv] != 0.0f;
     if (MK == 1) return gp(reinterpret_cast<const uint8_t*>(V.mask))[fv] != 0;
     return true;
 }
 
+// Foreground test of one full voxel (N4.py:280-287: mask and x > low_value).
+template <int MK>
+HD bool prep_fg(PrepMaskT<MK> m, float xv, const RunParams& rp) {
+    bool in;
+    if constexpr (MK == 3)
+        in = m > 0.0f && m < INFINITY;
+    else
+        in = m;
+    return in && xv > rp.low_f && isfinite(xv);
+}
+
+template <int MK>
+HD void prep_fold(PrepMaskT<MK> m, float xv, const RunParams& rp, int& nm, float& acc, float& wsum) {
+    if (prep_fg<MK>(m, xv, rp)) {
+        ++nm;
+        acc += logf(xv);
+        if constexpr (MK == 3) wsum += m;
+    }
+}
+
 template <int S, int MK>
-HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, int& nm, int& nb, float& acc) {
+HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, int& nm, int& nb, float& acc,
+                   float& wsum) {
     const int F0 = V.F[0], F1 = V.F[1], F2 = V.F[2];
     const int ss = S > 0 ? S : V.s;
     const int i0 = x * ss, j0 = J * ss, k0 = K * ss;
@@ -329,20 +359,17 @@ HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, i
         constexpr int SS = S > 0 ? S : 1;
         UNROLL for (int a = 0; a < SS; ++a) {
             float xv[SS * SS];
-            bool mv[SS * SS];
+            PrepMaskT<MK> mv[SS * SS];
             UNROLL for (int b = 0; b < SS; ++b) {
                 UNROLL for (int c = 0; c < SS; ++c) {
                     const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
                     xv[b * SS + c] = gp(V.im)[fv];
-                    mv[b * SS + c] = prep_mask<S, MK>(V, fv);
+                    mv[b * SS + c] = prep_mask<MK>(V, fv);
                 }
             }
             UNROLL for (int q = 0; q < SS * SS; ++q) {
                 ++nb;
-                if (mv[q] && xv[q] > rp.low_f && isfinite(xv[q])) {
-                    ++nm;
-                    acc += logf(xv[q]);
-                }
+                prep_fold<MK>(mv[q], xv[q], rp, nm, acc, wsum);
             }
         }
         return;
@@ -355,25 +382,24 @@ HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, i
             const size_t row = ((size_t)i * F1 + j) * F2;
             for (int k = k0; k < k1; ++k) {
                 ++nb;
-                const float xv = gp(V.im)[row + k];
-                if (prep_mask<S, MK>(V, row + k) && xv > rp.low_f && isfinite(xv)) {
-                    ++nm;
-                    acc += logf(xv);
-                }
+                prep_fold<MK>(prep_mask<MK>(V, row + k), gp(V.im)[row + k], rp, nm, acc, wsum);
             }
         }
 }
 
 // Writes one working voxel (block of nb full voxels, nm of them masked with
-// log sum acc) and folds it into the thread's partials.
-HD void prep_commit(const VolDesc& V, int x, int J, int K, int nm, int nb, float acc, double& nfull, double& nwork,
-                    double& mn, double& mx) {
+// log sum acc and, for confidence masks, weight sum wsum) and folds it into
+// the thread's partials.
+template <int MK>
+HD void prep_commit(const VolDesc& V, int x, int J, int K, int nm, int nb, float acc, float wsum, double& nfull,
+                    double& nwork, double& mn, double& mx) {
     const size_t w = ((size_t)x * V.W[1] + J) * V.W[2] + K;
     const bool fg = nm > 0 && 2 * nm >= nb;
     const float lg = nm > 0 ? acc / (float)nm : 0.0f;
     gp(V.imlog)[w] = lg;
     gp(V.logU)[w] = lg;
     gp(V.wmask)[w] = fg ? 1 : 0;
+    if constexpr (MK == 3) gp(V.wconf)[w] = fg ? wsum / (float)nm : 0.0f;
     nfull += nm;
     if (fg) {
         nwork += 1.0;
@@ -405,7 +431,7 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS
         const int n = sl.ny * W2;
         for (int v0 = t; v0 < n; v0 += UN * kBlock) {
             float xv[UN][NV];
-            bool mv[UN][NV];
+            PrepMaskT<MK> mv[UN][NV];
             bool in[UN];
             UNROLL for (int u = 0; u < UN; ++u) {
                 const int v = v0 + u * kBlock;
@@ -418,7 +444,7 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS
                         UNROLL for (int a = 0; a < SB; ++a) UNROLL for (int b = 0; b < SB; ++b) UNROLL for (int c = 0; c < SB; ++c) {
                             const size_t fv = ((size_t)(i0 + a) * F1 + (j0 + b)) * F2 + (k0 + c);
                             xv[u][(a * SB + b) * SB + c] = gp(V.im)[fv];
-                            mv[u][(a * SB + b) * SB + c] = prep_mask<S, MK>(V, fv);
+                            mv[u][(a * SB + b) * SB + c] = prep_mask<MK>(V, fv);
                         }
                     }
                 }
@@ -429,19 +455,16 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS
                 const int jr = (int)fdiv((uint32_t)v, V.mW2);
                 const int J = sl.y0 + jr, K = v - jr * W2;
                 int nm = 0, nb = 0;
-                float acc = 0.0f;
+                float acc = 0.0f, wsum = 0.0f;
                 if (in[u]) {
                     UNROLL for (int q = 0; q < NV; ++q) {
                         ++nb;
-                        if (mv[u][q] && xv[u][q] > rp.low_f && isfinite(xv[u][q])) {
-                            ++nm;
-                            acc += logf(xv[u][q]);
-                        }
+                        prep_fold<MK>(mv[u][q], xv[u][q], rp, nm, acc, wsum);
                     }
                 } else {
-                    prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
+                    prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc, wsum);
                 }
-                prep_commit(V, x, J, K, nm, nb, acc, nfull, nwork, mn, mx);
+                prep_commit<MK>(V, x, J, K, nm, nb, acc, wsum, nfull, nwork, mn, mx);
             }
         }
     } else {
@@ -449,9 +472,9 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS
             const int jr = (int)fdiv((uint32_t)v, V.mW2);
             const int J = sl.y0 + jr, K = v - jr * W2;
             int nm = 0, nb = 0;
-            float acc = 0.0f;
-            prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc);
-            prep_commit(V, x, J, K, nm, nb, acc, nfull, nwork, mn, mx);
+            float acc = 0.0f, wsum = 0.0f;
+            prep_voxel<S, MK>(V, rp, x, J, K, nm, nb, acc, wsum);
+            prep_commit<MK>(V, x, J, K, nm, nb, acc, wsum, nfull, nwork, mn, mx);
         }
     }
     red[t] = nfull;
@@ -473,7 +496,9 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS
 
 template <int S>
 HD void body_prep_s(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS unsigned char* smem) {
-    if (V.mask_kind == 0)
+    if (V.wconf)
+        body_prep_t<S, 3>(V, sl, rp, smem);
+    else if (V.mask_kind == 0)
         body_prep_t<S, 0>(V, sl, rp, smem);
     else if (V.mask_kind == 1)
         body_prep_t<S, 1>(V, sl, rp, smem);
@@ -500,6 +525,24 @@ HD double refine_w(int odd, int a) {
     return odd ? (a == 0 ? 0.0 : 0.5) : (a == 1 ? 0.75 : 0.125);
 }
 
+// Run records of a correction (both nullable in VolDesc): the convergence
+// value of iteration it (1-based) of level lvl, at index sum of the earlier
+// levels' maxima + it - 1 of cvtrace, and the FP32 lattice of each level at
+// its end, at offset sum of the earlier levels' sizes of lvlat.
+HD void record_cv(const VolDesc& V, int lvl, int it, double cvv) {
+    if (!V.cvtrace) return;
+    int o = it - 1;
+    for (int l = 0; l < lvl; ++l) o += V.max_iter[l];
+    if (o < V.trace_cap) V.cvtrace[o] = cvv;
+}
+
+HD float* level_lattice_record(const VolDesc& V, int lvl) {
+    if (!V.lvlat) return nullptr;
+    size_t o = 0;
+    for (int l = 0; l < lvl; ++l) o += (size_t)V.lev[l].L;
+    return V.lvlat + o;
+}
+
 // Lattice of `level` (> 0) as the dyadic refinement of level-1's FP64
 // lattice (N4.py:132-165, cudaN4.cu:257-339) with per-axis clip bounds;
 // also its FP32 copy.  Runs inside a thread phase (thread t of kBlock).
@@ -511,6 +554,8 @@ HD void refine_lattice(const VolDesc& V, int level, int t) {
     const LevelDesc& PD = V.lev[level - 1];
     const double* src = V.lat[(level - 1) & 1];
     double* dst = V.lat[level & 1];
+    if (float* rec = level_lattice_record(V, level - 1))
+        for (int l = t; l < PD.L; l += kBlock) rec[l] = (float)(TEAM ? ld_agent(src + l) : src[l]);
     for (int l = t; l < LD.L; l += kBlock) {
         int c[3];
         c[0] = l / (LD.C[1] * LD.C[2]);
@@ -753,6 +798,7 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     if (sl.x == 0 && sl.y0 == 0) {
         BLOCK_FOR(t)
         if (t == 0) {
+            if (mode == kModeIter && it > 0 && step >= 0) record_cv(V, lvl, it, cvv);
             if (level_end) {
                 st->iters[lvl] = it;
                 if (it > 0) {
@@ -1007,8 +1053,12 @@ HD void volume_step(const VolDesc& V, int kind, int s_lvl, int s_it, bool degene
     } else if (kind == kStepFinal) {
         const LevelDesc& FD = V.lev[V.nlevels - 1];
         const double* fin = V.lat[(V.nlevels - 1) & 1];
+        float* rec = level_lattice_record(V, V.nlevels - 1);
         BLOCK_FOR(t)
-        for (int l = t; l < FD.L; l += kBlock) V.latf[l] = (float)fin[l];
+        for (int l = t; l < FD.L; l += kBlock) {
+            V.latf[l] = (float)fin[l];
+            if (rec) rec[l] = (float)fin[l];
+        }
         if (t == 0) {
             st->mode = kModeFinished;
             st->done = 1;
@@ -1177,7 +1227,10 @@ HD Remap make_remap(const N4_LDS float* E, double bmin, double slope, int nb) {
 
 // Contracted quantity g of one working voxel: the residual logU - E[u|v]
 // (N4.py:103-115 remap with the cudaN4.cu:242 clamp, then N4.py:313) for
-// NUM, the mask for DEN, logU itself for FIELD; 0 outside the mask.
+// NUM, the mask for DEN, logU itself for FIELD; 0 outside the mask.  With
+// confidence weights (V.wconf), the fit kernels scale g of a foreground
+// voxel by its weight (conf_scale), which scales both its numerator and its
+// denominator contribution.
 HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const Remap& R) {
     if (!m) return 0.0f;
     if (mode == kFitDen) return 1.0f;
@@ -1189,6 +1242,10 @@ HD float fit_input(int mode, uint8_t m, float xv, bool degenerate, const Remap& 
     const float fr = c - (float)idx;
     const float sh = (idx < R.nb - 1) ? R.E[idx] + (R.E[idx + 1] - R.E[idx]) * fr : R.E[R.nb - 1];
     return xv - sh;
+}
+
+HD float conf_scale(const VolDesc& V, uint8_t m, size_t w, float g) {
+    return (V.wconf && m) ? g * gp(V.wconf)[w] : g;
 }
 
 HD size_t carve_fit(N4_LDS unsigned char* base, int nbins, int Y, int Z, int R, int Cy, int Cz, FitLds* o) {
@@ -1278,7 +1335,7 @@ HD void fit_chunks(const VolDesc& V, const LevelDesc& LD, const Slab sl, const F
                     const size_t w = wq + j;
                     if (w < base || w >= vend) continue;
                     const uint8_t m = (uint8_t)((ms[u] >> (8 * j)) & 0xffu);
-                    S.g[w - base] = fit_input(gmode, m, xs[u].v[j], degenerate, rm);
+                    S.g[w - base] = conf_scale(V, m, w, fit_input(gmode, m, xs[u].v[j], degenerate, rm));
                 }
             }
         }
@@ -2171,6 +2228,7 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
         if (leader) {
             BLOCK_FOR(t)
             if (t == 0) {
+                if (mode == kModeIter && it > 0) record_cv(V, lvl, it, cvv);
                 if (level_end) {
                     st->iters[lvl] = it;
                     if (it > 0) {
@@ -2195,8 +2253,13 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
             if (leader) {
                 const LevelDesc& FD = V.lev[V.nlevels - 1];
                 const double* fin = V.lat[(V.nlevels - 1) & 1];
+                float* rec = level_lattice_record(V, V.nlevels - 1);
                 BLOCK_FOR(t)
-                for (int l = t; l < FD.L; l += kBlock) V.latf[l] = (float)ld_agent(fin + l);
+                for (int l = t; l < FD.L; l += kBlock) {
+                    const float f = (float)ld_agent(fin + l);
+                    V.latf[l] = f;
+                    if (rec) rec[l] = f;
+                }
                 if (t == 0) {
                     st->mode = kModeFinished;
                     st->done = 1;
@@ -2584,6 +2647,40 @@ int launch_recon(const VolDesc* vols, const Slab* slabs, int nslab, int level, R
     return last_launch_status();
 }
 
+// Float mask -> uint8 mask (nonzero = foreground, cudaN4.h:346),
+// grid-stride.  vec (in 16-byte and out 4-byte aligned): four voxels per
+// thread and iteration, one 16-byte load and one 4-byte store, the last
+// n % 4 voxels by block 0; else one voxel per thread and iteration.
+__global__ __launch_bounds__(kBlock) void k_mask_u8(const float* __restrict__ in, uint8_t* __restrict__ out,
+                                                    uint64_t n, int vec) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (!vec) {
+        for (uint64_t i = t0; i < n; i += stride) gp(out)[i] = gp(in)[i] != 0.0f ? 1 : 0;
+        return;
+    }
+    const uint64_t nq = n >> 2;
+    for (uint64_t q = t0; q < nq; q += stride) {
+        const F4 v = *reinterpret_cast<const N4_GLOBAL F4*>(gp(in) + 4 * q);
+        const uint32_t w = (v.v[0] != 0.0f ? 1u : 0u) | (v.v[1] != 0.0f ? 1u << 8 : 0u) |
+                           (v.v[2] != 0.0f ? 1u << 16 : 0u) | (v.v[3] != 0.0f ? 1u << 24 : 0u);
+        *reinterpret_cast<N4_GLOBAL uint32_t*>(gp(out) + 4 * q) = w;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+        const uint64_t i = 4 * nq + threadIdx.x;
+        gp(out)[i] = gp(in)[i] != 0.0f ? 1 : 0;
+    }
+}
+
+int launch_mask_u8(const float* in, uint8_t* out, uint64_t n, void* stream) {
+    if (n == 0) return 0;
+    const int vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 4 == 0);
+    const uint64_t items = vec ? std::max<uint64_t>(n >> 2, 1) : n;
+    const uint64_t blocks = std::min<uint64_t>((items + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_mask_u8, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream, in, out, n, vec);
+    return last_launch_status();
+}
+
 int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const float* a, const float* b,
                 float* out, RunParams rp, void* stream) {
     if (nslab <= 0) return 0;
@@ -2688,6 +2785,30 @@ int launch_flat(const VolDesc* vols, const Slab* slabs, int nslab, int op, const
     for (int i = 0; i < nslab; ++i) body_flat(vols, slabs, op, a, b, out, rp, i, buf);
     return 0;
 }
+
+int launch_mask_u8(const float* in, uint8_t* out, uint64_t n, void*) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = in[i] != 0.0f ? 1 : 0;
+    return 0;
+}
+
+}  // namespace n4
+
+// Test hook of the emulation build only: the FP32 remap of the correction
+// loop (fit_input, NUM mode: residual x - E[u|x] over the masked voxels,
+// 0 elsewhere) on n values, with the FP64 E table narrowed to FP32 as the
+// fit kernels stage it in LDS.  The CPU tests compare it with the oracle's
+// FP64 residual at bin edges, duplicated maxima and values below bmin.
+extern "C" int n4emu_remap_residual(const float* x, const uint8_t* m, int64_t n, const double* E, int nb,
+                                    double bmin, double slope, int degenerate, float* out) {
+    if (nb < 2 || nb > n4::kMaxBins) return 1;
+    float Ef[n4::kMaxBins];
+    for (int i = 0; i < nb; ++i) Ef[i] = (float)E[i];
+    const n4::Remap rm = n4::make_remap(Ef, bmin, slope, nb);
+    for (int64_t i = 0; i < n; ++i) out[i] = n4::fit_input(n4::kFitNum, m[i], x[i], degenerate != 0, rm);
+    return 0;
+}
+
+namespace n4 {
 
 #endif
 

@@ -35,6 +35,7 @@ n4_handle* thread_handle() {
 }
 
 thread_local n4_report g_last_report;
+thread_local unsigned g_last_levels = 0;  // fitting levels of that call
 
 // Device buffer owned for the duration of a call.
 struct DevBuf {
@@ -48,13 +49,9 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-// float mask (nonzero = foreground, cudaN4.h:346) -> uint8 mask on the device.
+// float mask (nonzero = foreground, cudaN4.h:346) -> uint8 mask, on the device.
 void mask_to_u8(n4_handle* h, const float* d_mask, size_t n, void* d_out) {
-    std::vector<float> m(n);
-    std::vector<uint8_t> u(n);
-    check(n4_memcpy_dtoh(h, m.data(), d_mask, n * sizeof(float)), "mask download");
-    for (size_t i = 0; i < n; ++i) u[i] = m[i] != 0.0f;
-    check(n4_memcpy_htod(h, d_out, u.data(), n), "mask upload");
+    check(n4_mask_to_u8(h, d_mask, n, (uint8_t*)d_out), "n4_mask_to_u8");
 }
 
 double widen(float f) {
@@ -84,6 +81,7 @@ n4_params N4ParamToC(const N4Param& q) {
     p.shrink_factor = q.ShrinkFactor;
     p.fit_eps = q.FitEps;
     p.precision = q.Precision;
+    p.flags = (q.FixedIterations ? N4_FLAG_FIXED_ITERATIONS : 0u) | (q.WeightedMask ? N4_FLAG_WEIGHTED_MASK : 0u);
     return p;
 }
 
@@ -248,6 +246,7 @@ void N4(float* d_im, float* d_mask, const unsigned int fx, const unsigned int fy
     check(n4_correct(data_handle(data), d_im, d_mask, d_mask ? N4_MASK_F32 : N4_MASK_NONE, fx, fy, fz, &p,
                      d_im_normalized, d_lattice, d_biasField, nullptr, &g_last_report, nullptr),
           "n4_correct");
+    g_last_levels = p.number_of_fitting_levels;
 }
 
 void testN4(float* h_im, float* h_mask, const unsigned int fx, const unsigned int fy, const unsigned int fz,
@@ -256,9 +255,36 @@ void testN4(float* h_im, float* h_mask, const unsigned int fx, const unsigned in
     check(n4_correct_host(data_handle(data), h_im, h_mask, fx, fy, fz, &p, h_im_normalized, h_lattice,
                           h_biasField, &g_last_report),
           "n4_correct_host");
+    g_last_levels = p.number_of_fitting_levels;
 }
 
 const n4_report& N4LastReport() { return g_last_report; }
+
+std::vector<std::vector<double>> N4LastCVTrace(N4Data& data) {
+    n4_handle* h = data_handle(data);
+    uint32_t n = 0;
+    check(n4_last_cv_trace(h, 0, nullptr, 0, &n), "n4_last_cv_trace");
+    std::vector<double> all(n);
+    if (n) check(n4_last_cv_trace(h, 0, all.data(), n, &n), "n4_last_cv_trace");
+    std::vector<std::vector<double>> per;
+    size_t k = 0;
+    for (unsigned l = 0; l < g_last_levels; ++l) {
+        const size_t it = std::min<size_t>(g_last_report.iterations[l], all.size() - k);
+        per.emplace_back(all.begin() + k, all.begin() + std::min(all.size(), k + it));
+        k += it;
+    }
+    return per;
+}
+
+std::vector<float> N4LastLevelLattice(N4Data& data, unsigned level) {
+    // level l of L has (final - 3) / 2^(L-1-l) + 3 control points per axis
+    if (level >= g_last_levels) throw n4_error("N4LastLevelLattice: level out of range");
+    size_t n = 1;
+    for (int d = 0; d < 3; ++d) n *= ((g_last_report.final_ncpt[d] - 3) >> (g_last_levels - 1 - level)) + 3;
+    std::vector<float> out(n);
+    check(n4_last_level_lattice(data_handle(data), 0, level, out.data(), n), "n4_last_level_lattice");
+    return out;
+}
 
 float Reducer::reduce_sum_wrapper(int n, float* d_idata, float*) {
     double s = 0.0;

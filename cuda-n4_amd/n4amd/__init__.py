@@ -26,6 +26,7 @@ MASK_F32, MASK_U8, MASK_NONE = 0, 1, 2
 PREC_FP32, PREC_BF16, PREC_SPLIT_BF16 = 0, 1, 2
 PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "split_bf16": PREC_SPLIT_BF16}
 FLAG_FIXED_ITERATIONS = 0x1
+FLAG_WEIGHTED_MASK = 0x2
 
 # Names every build of the library must export (include/n4_capi.h).
 EXPORTS = [
@@ -35,7 +36,8 @@ EXPORTS = [
     "n4_convergence_measurement", "n4_upsample_lattice3d", "n4_reduce_sum", "n4_reduce_max",
     "n4_reduce_min", "n4_device_count", "n4_malloc", "n4_free", "n4_memcpy_htod",
     "n4_memcpy_dtoh", "n4_memcpy_dtod", "n4_memset", "n4_synchronize", "n4_profile_enable",
-    "n4_profile_read", "n4_get_tuning", "n4_set_tuning", "n4_autotune_batch",
+    "n4_profile_read", "n4_get_tuning", "n4_set_tuning", "n4_autotune_batch", "n4_last_cv_trace",
+    "n4_last_level_lattice", "n4_mask_to_u8",
 ]
 KERNELS = ["prep", "level", "hist", "fit_den", "fit", "eval", "recon", "wiener", "lattice", "update", "teams",
            "t_wait", "t_reduce", "t_hist", "t_wiener", "t_fit", "t_update", "t_eval", "t_trans"]
@@ -156,6 +158,9 @@ def _declare(L):
         "n4_get_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
         "n4_set_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
         "n4_autotune_batch": ([vp, C.POINTER(Volume), u32, C.POINTER(Params), C.POINTER(Tuning), vp], C.c_int),
+        "n4_last_cv_trace": ([vp, u32, vp, u32, C.POINTER(u32)], C.c_int),
+        "n4_last_level_lattice": ([vp, u32, u32, vp, u64], C.c_int),
+        "n4_mask_to_u8": ([vp, vp, u64, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -189,7 +194,8 @@ def default_params(**kw) -> Params:
     """n4_params with the reference defaults; keyword overrides:
     nbins, noise, fwhm, ncpt (int or 3-tuple), levels, max_iter (int or list),
     threshold, low_value, shrink, eps, precision ("fp32", "bf16",
-    "split_bf16" or the n4_precision value), fixed_iterations."""
+    "split_bf16" or the n4_precision value), fixed_iterations,
+    weighted_mask (float masks are confidence weights)."""
     p = Params()
     p.number_of_histogram_bins = 200
     p.wiener_filter_noise = 0.01
@@ -219,8 +225,9 @@ def default_params(**kw) -> Params:
                 p.max_iterations[i] = int(v[i]) if i < len(v) else int(v[-1])
         elif k == "precision":
             p.precision = PRECISIONS[v] if isinstance(v, str) else int(v)
-        elif k == "fixed_iterations":
-            p.flags = (p.flags | FLAG_FIXED_ITERATIONS) if v else (p.flags & ~FLAG_FIXED_ITERATIONS)
+        elif k in ("fixed_iterations", "weighted_mask"):
+            f = FLAG_FIXED_ITERATIONS if k == "fixed_iterations" else FLAG_WEIGHTED_MASK
+            p.flags = (p.flags | f) if v else (p.flags & ~f)
         else:
             name = alias.get(k, k)
             if name not in {f[0] for f in Params._fields_}:
@@ -340,11 +347,32 @@ class Library:
         return best
 
     # ------------------------------------------------------------ N4 ----
+    def last_cv_trace(self, volume=0):
+        """Convergence value of every iteration of volume `volume` of the
+        last correction call, level after level (n4_last_cv_trace)."""
+        n = C.c_uint32(0)
+        self._check(self.L.n4_last_cv_trace(self.h, volume, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        if n.value:
+            self._check(self.L.n4_last_cv_trace(self.h, volume, out.ctypes.data_as(C.c_void_p), n.value,
+                                                C.byref(n)))
+        return out
+
+    def last_level_lattice(self, volume, level, ncpt0):
+        """FP32 lattice of fitting level `level` at that level's end
+        (n4_last_level_lattice); ncpt0: the first level's control points."""
+        nc0 = (ncpt0,) * 3 if np.isscalar(ncpt0) else tuple(ncpt0)
+        shape = tuple((int(c) - 3) * 2 ** level + 3 for c in nc0)
+        out = np.zeros(shape, np.float32)
+        self._check(self.L.n4_last_level_lattice(self.h, volume, level, out.ctypes.data_as(C.c_void_p), out.size))
+        return out
+
     def correct(self, im, mask=None, params: Params | None = None, want_bias=False, want_log_bias=True,
-                mask_kind=None, noalloc=False, **kw):
+                mask_kind=None, noalloc=False, want_levels=False, **kw):
         """Whole correction of one volume.  Returns dict with normalized,
-        log_bias (optional), bias (optional), lattice, report.  noalloc: the
-        allocation-free entry point (n4_correct_noalloc)."""
+        log_bias (optional), bias (optional), lattice, report, cv_trace and,
+        with want_levels, each level's lattice.  noalloc: the allocation-free
+        entry point (n4_correct_noalloc)."""
         p = params if params is not None else default_params(**kw)
         im = np.ascontiguousarray(im, dtype=np.float32)
         dims = im.shape
@@ -376,6 +404,10 @@ class Library:
             "n_mask_work": int(rep.n_mask_work),
             "elapsed_ms": float(rep.elapsed_ms),
         }
+        out["cv_trace"] = self.last_cv_trace(0)
+        if want_levels:
+            out["level_lattices"] = [self.last_level_lattice(0, l, tuple(p.ncpt))
+                                     for l in range(p.number_of_fitting_levels)]
         if d_bias is not None:
             out["bias"] = d_bias.download(np.float32, dims)
         if d_lb is not None:
@@ -388,7 +420,8 @@ class Library:
         mask goes to the device as N4_MASK_U8, any other dtype as float32).
         outputs: indices of the volumes whose arrays are downloaded and
         whose log bias is computed (None: all); the others report their
-        iteration counts only."""
+        iteration counts only.  Per-volume CV traces and level lattices:
+        last_cv_trace(i) / last_level_lattice(i, level, ncpt) afterwards."""
         p = params if params is not None else default_params(**kw)
         nc = final_ncpt(p)
         sel = None if outputs is None else set(int(i) for i in outputs)
@@ -461,6 +494,17 @@ class Library:
         de = self.empty(8 * nbins)
         self._check(self.L.n4_sharpen_image(self.h, dx.ptr, dm.ptr, x.size, nbins, noise, fwhm, do.ptr, de.ptr))
         return do.download(np.float32, shape), de.download(np.float64, nbins)
+
+    def mask_to_u8(self, mask, offset=0):
+        """Float mask -> uint8 mask on the device (n4_mask_to_u8); offset
+        (elements) shifts both device pointers, to exercise unaligned
+        buffers."""
+        m = np.ascontiguousarray(mask, dtype=np.float32).ravel()
+        src = np.concatenate([np.zeros(offset, np.float32), m])
+        dm = self.to_device(src)
+        do = self.empty(m.size + offset + 4)
+        self._check(self.L.n4_mask_to_u8(self.h, dm.ptr + 4 * offset, m.size, do.ptr + offset))
+        return do.download(np.uint8, (m.size + offset + 4,))[offset:offset + m.size]
 
     def fit_bspline3d(self, field, mask, ncpt, eps=1e-3):
         field = np.ascontiguousarray(field, dtype=np.float32)

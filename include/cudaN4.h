@@ -16,7 +16,10 @@
  *     the convergence measure uses ddof 0 (N4.py:125) and the level loop
  *     stops once it is <= ConvergenceThreshold (N4.py:301-302);
  *   - extra fields at the end of N4Param: ShrinkFactor, FitEps,
- *     IterationsPerLevel (per-level maxima, N4.py:278), Precision.
+ *     IterationsPerLevel (per-level maxima, N4.py:278), Precision,
+ *     FixedIterations (run every level to its maximum, the reference's
+ *     effective behaviour, cudaN4.cu:1643), WeightedMask (float masks are
+ *     confidence weights; the reference takes binary masks, cudaN4.h:346).
  * Errors throw n4_error (std::runtime_error) instead of exiting.
  */
 #ifndef CUDAN4_H
@@ -58,6 +61,8 @@ struct N4Param {
     double FitEps = 1e-3;
     std::vector<unsigned int> IterationsPerLevel;    /* empty: MaximumNumberOfIterations */
     unsigned int Precision = N4_PREC_FP32;
+    bool FixedIterations = false;                    /* N4_FLAG_FIXED_ITERATIONS */
+    bool WeightedMask = false;                       /* N4_FLAG_WEIGHTED_MASK */
 };
 
 /* Fills C-ABI parameters from N4Param.  Float fields are widened through
@@ -126,5 +131,13 @@ void testN4(float* h_im, float* h_mask, const unsigned int fx, const unsigned in
 
 /* Per-call run record of the last testN4/N4 on this thread (extension). */
 const n4_report& N4LastReport();
+
+/* Convergence value of every iteration of the last testN4/N4 on `data`,
+ * one vector per level (extension; n4_last_cv_trace). */
+std::vector<std::vector<double>> N4LastCVTrace(N4Data& data);
+
+/* Lattice of fitting level `level` at that level's end, of the last
+ * testN4/N4 on `data`, [cx][cy][cz] (extension; n4_last_level_lattice). */
+std::vector<float> N4LastLevelLattice(N4Data& data, unsigned level);
 
 #endif

@@ -45,6 +45,14 @@ typedef enum n4_precision {
 
 /* Flags for n4_params.flags */
 #define N4_FLAG_FIXED_ITERATIONS 0x1u /* ignore the convergence threshold */
+/* F32 masks are confidence weights (the reference takes binary masks only,
+ * cudaN4.h:346): a voxel is foreground where its weight is > 0 and finite;
+ * its fit contributions (numerator and denominator, N4.py:230-236) are
+ * scaled by the weight, as ITK/ANTs N4 uses a confidence image; histogram
+ * and convergence measure stay unweighted over the foreground.  With a
+ * shrink factor the working voxel's weight is the mean weight of the
+ * foreground voxels of its block.  U8 and absent masks stay binary. */
+#define N4_FLAG_WEIGHTED_MASK 0x2u
 
 /* Parameters.  Replaces struct N4Param (cudaN4.h:24-94); N4.py:276-279,
  * 292, 307-311 for the per-level iteration list and fixed constants.
@@ -150,6 +158,27 @@ n4_status n4_correct_host(n4_handle *h, const float *h_im, const float *h_mask,
 n4_status n4_correct_batch(n4_handle *h, const n4_volume *vols, uint32_t nvol,
                            const n4_params *p, n4_report *reports, void *stream);
 
+/* Run records of the last correction call on this handle (n4_correct,
+ * n4_correct_noalloc, n4_correct_host, n4_correct_batch), volume `volume`
+ * of that call.  The reference prints "Level %d, iter %d" per iteration
+ * (cudaN4.cu:1530) and drops the lattice (main.cpp:122); these keep both.
+ *   n4_last_cv_trace: the convergence measurement (N4.py:121-126) of every
+ *     iteration, level after level (report.iterations[l] values for level
+ *     l); *count = the number of values, of which the first `capacity` are
+ *     written.  Iterations after the first 4096 of a call are not recorded.
+ *   n4_last_level_lattice: the FP32 lattice of fitting level `level` as it
+ *     stood at the end of that level, [cx][cy][cz] with cz fastest,
+ *     (ncpt[d] - 3) * 2^level + 3 control points along axis d; capacity in
+ *     floats.
+ * Both fail with N4_ERR_INVALID_ARG once the record is gone: any later call
+ * on the handle (stage calls included) reuses its memory, and so does, for
+ * a record of n4_correct_noalloc, any later allocation-free call on the
+ * device.  Synchronous. */
+n4_status n4_last_cv_trace(n4_handle *h, uint32_t volume, double *out,
+                           uint32_t capacity, uint32_t *count);
+n4_status n4_last_level_lattice(n4_handle *h, uint32_t volume, uint32_t level,
+                                float *out, uint64_t capacity);
+
 /* ---- stage entry points (device pointers, synchronous) ----------------- */
 /* Masked min/max + fractional histogram (cudaN4.cu:703-731).
  * d_mask: uint8.  out_minmax_slope (host, 3 doubles), d_hist (nbins doubles,
@@ -184,6 +213,13 @@ n4_status n4_convergence_measurement(n4_handle *h, const float *d_field1,
                                      const float *d_field2,
                                      const uint8_t *d_mask, uint64_t n,
                                      double *out);
+
+/* The reference's float masks (nonzero = foreground, cudaN4.h:346) as the
+ * uint8 masks the stage entry points take: d_out[i] = d_mask[i] != 0, on
+ * the device (n values; synchronous).  No counterpart in the reference,
+ * whose kernels read the float mask directly. */
+n4_status n4_mask_to_u8(n4_handle *h, const float *d_mask, uint64_t n,
+                        uint8_t *d_out);
 
 /* Replaces upsampleLattice3D_gpu (cudaN4.h:269-276): (n+3) -> (2n+3). */
 n4_status n4_upsample_lattice3d(n4_handle *h, const float *d_lattice,

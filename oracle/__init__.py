@@ -35,6 +35,7 @@ class Params(C.Structure):
         ("shrink", C.c_int32),
         ("fit_eps", C.c_double),
         ("nthreads", C.c_int32),
+        ("weighted", C.c_int32),
     ]
 
 
@@ -78,6 +79,7 @@ def lib():
                                   _f64p, _f64p, _f64p]
         L.n4o_fit_bspline3d.argtypes = [_f64p, _u8p, _i32p, _i32p, C.c_double, C.c_int32, _f64p]
         L.n4o_fit_numden.argtypes = [_f64p, _u8p, _i32p, _i32p, C.c_int32, _f64p, _f64p]
+        L.n4o_fit_numden_w.argtypes = [_f64p, _u8p, C.c_void_p, _i32p, _i32p, C.c_int32, _f64p, _f64p]
         L.n4o_evaluate_bspline3d.argtypes = [_f64p, _i32p, _i32p, _f64p, C.c_int32, _f64p]
         L.n4o_convergence.argtypes = [_f64p, _f64p, _u8p, C.c_int64]
         L.n4o_convergence.restype = C.c_double
@@ -85,6 +87,9 @@ def lib():
         L.n4o_threshold_log_shrink.argtypes = [_f32p, C.c_void_p, _i32p, C.c_double, C.c_int32, _i32p,
                                                C.c_void_p, C.c_void_p, C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64)]
+        L.n4o_threshold_log_shrink_w.argtypes = [_f32p, C.c_void_p, C.c_int32, _i32p, C.c_double, C.c_int32,
+                                                 _i32p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.n4o_n4.argtypes = [_f32p, C.c_void_p, _i32p, C.POINTER(Params), C.c_void_p, C.c_void_p,
                              C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Report)]
         L.n4o_n4.restype = C.c_int32
@@ -190,13 +195,20 @@ def fit_bspline3d(field, mask, ncpt, eps=1e-3, nthreads=1):
     return lat
 
 
-def fit_numden(field, mask, ncpt, nthreads=1):
+def fit_numden(field, mask, ncpt, nthreads=1, conf=None):
+    """Numerator and denominator of the fit; conf (optional) holds per-voxel
+    confidence weights."""
     field = _c64(field)
     dims = np.array(field.shape, np.int32)
     nc = np.array(ncpt, np.int32)
     num = np.zeros(tuple(nc))
     den = np.zeros(tuple(nc))
-    lib().n4o_fit_numden(field.ravel(), _mask_u8(mask).ravel(), dims, nc, nthreads, num.ravel(), den.ravel())
+    if conf is None:
+        lib().n4o_fit_numden(field.ravel(), _mask_u8(mask).ravel(), dims, nc, nthreads, num.ravel(), den.ravel())
+    else:
+        cf = _c64(conf)
+        lib().n4o_fit_numden_w(field.ravel(), _mask_u8(mask).ravel(), cf.ctypes.data_as(C.c_void_p), dims, nc,
+                               nthreads, num.ravel(), den.ravel())
     return num, den
 
 
@@ -224,18 +236,24 @@ def upsample_lattice3d(lattice):
     return out
 
 
-def threshold_log_shrink(im, mask=None, low_value=10.0, shrink=1):
+def threshold_log_shrink(im, mask=None, low_value=10.0, shrink=1, weighted=False):
+    """Working-grid log image and mask (and, with weighted=True, the working
+    voxels' mean confidence weights as a fifth output)."""
     im = np.ascontiguousarray(im, dtype=np.float32)
     fd = np.array(im.shape, np.int32)
     wd = np.zeros(3, np.int32)
     W = [(int(f) + shrink - 1) // shrink for f in im.shape]
     imlog = np.zeros(W)
     mw = np.zeros(W, np.uint8)
+    cw = np.zeros(W)
     mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
     nf, nw = C.c_int64(), C.c_int64()
-    lib().n4o_threshold_log_shrink(im.ravel(), None if mk is None else mk.ctypes.data_as(C.c_void_p), fd,
-                                   low_value, shrink, wd, imlog.ctypes.data_as(C.c_void_p),
-                                   mw.ctypes.data_as(C.c_void_p), C.byref(nf), C.byref(nw))
+    lib().n4o_threshold_log_shrink_w(im.ravel(), None if mk is None else mk.ctypes.data_as(C.c_void_p),
+                                     int(bool(weighted)), fd, low_value, shrink, wd,
+                                     imlog.ctypes.data_as(C.c_void_p), mw.ctypes.data_as(C.c_void_p),
+                                     cw.ctypes.data_as(C.c_void_p), C.byref(nf), C.byref(nw))
+    if weighted:
+        return imlog, mw, nf.value, nw.value, cw
     return imlog, mw, nf.value, nw.value
 
 

@@ -290,10 +290,12 @@ static void free_tabs(axis_tab t[3]) {
     }
 }
 
-/* N4.py:186-237 direct form; accumulates numerator and denominator. */
-void n4o_fit_numden(const double *field, const uint8_t *mask,
-                    const int32_t dims[3], const int32_t ncpt[3],
-                    int32_t nthreads, double *num, double *den) {
+/* N4.py:186-237 direct form; accumulates numerator and denominator, each
+ * voxel's terms scaled by its confidence weight conf[v] (NULL: 1). */
+void n4o_fit_numden_w(const double *field, const uint8_t *mask,
+                      const double *conf, const int32_t dims[3],
+                      const int32_t ncpt[3], int32_t nthreads, double *num,
+                      double *den) {
     const int32_t X = dims[0], Y = dims[1], Z = dims[2];
     const int64_t L = (int64_t)ncpt[0] * ncpt[1] * ncpt[2];
     double dom[3] = {X, Y, Z};
@@ -329,14 +331,15 @@ void n4o_fit_numden(const double *field, const uint8_t *mask,
                             wc_sum += w * w;
                         }
                 double f = field[v];
+                const double cw = conf ? conf[v] : 1.0;
                 for (int a = 0; a < 4; ++a)
                     for (int b = 0; b < 4; ++b)
                         for (int c = 0; c < 4; ++c) {
                             double w = wc[a * 16 + b * 4 + c];
                             double phi = f * w / wc_sum;
                             int64_t li = ((int64_t)(bx + a) * ncpt[1] + (by + b)) * ncpt[2] + (bz + c);
-                            an[li] += (w * w) * phi;
-                            ad[li] += w * w;
+                            an[li] += cw * ((w * w) * phi);
+                            ad[li] += cw * (w * w);
                         }
             }
         }
@@ -355,17 +358,30 @@ void n4o_fit_numden(const double *field, const uint8_t *mask,
     free_tabs(t);
 }
 
+void n4o_fit_numden(const double *field, const uint8_t *mask,
+                    const int32_t dims[3], const int32_t ncpt[3],
+                    int32_t nthreads, double *num, double *den) {
+    n4o_fit_numden_w(field, mask, NULL, dims, ncpt, nthreads, num, den);
+}
+
 /* N4.py:236: lattice = wc2_phic/(wc2 + eps) */
-void n4o_fit_bspline3d(const double *field, const uint8_t *mask,
-                       const int32_t dims[3], const int32_t ncpt[3],
-                       double eps, int32_t nthreads, double *lattice) {
+static void fit_bspline3d_w(const double *field, const uint8_t *mask,
+                            const double *conf, const int32_t dims[3],
+                            const int32_t ncpt[3], double eps,
+                            int32_t nthreads, double *lattice) {
     const int64_t L = (int64_t)ncpt[0] * ncpt[1] * ncpt[2];
     double *num = malloc(sizeof(double) * L);
     double *den = malloc(sizeof(double) * L);
-    n4o_fit_numden(field, mask, dims, ncpt, nthreads, num, den);
+    n4o_fit_numden_w(field, mask, conf, dims, ncpt, nthreads, num, den);
     for (int64_t l = 0; l < L; ++l) lattice[l] = num[l] / (den[l] + eps);
     free(num);
     free(den);
+}
+
+void n4o_fit_bspline3d(const double *field, const uint8_t *mask,
+                       const int32_t dims[3], const int32_t ncpt[3],
+                       double eps, int32_t nthreads, double *lattice) {
+    fit_bspline3d_w(field, mask, NULL, dims, ncpt, eps, nthreads, lattice);
 }
 
 /* N4.py:241-273 */
@@ -460,12 +476,15 @@ void n4o_upsample_lattice3d(const double *lat, const int32_t ncpt[3],
             }
 }
 
-/* N4.py:280-287 (threshold, log) + the build-defined shrink. */
-void n4o_threshold_log_shrink(const float *im, const float *mask_f,
-                              const int32_t fdims[3], double low_value,
-                              int32_t s, int32_t wdims[3], double *imlog_w,
-                              uint8_t *mask_w, int64_t *n_mask_full,
-                              int64_t *n_mask_work) {
+/* N4.py:280-287 (threshold, log) + the build-defined shrink; with
+ * weighted != 0, mask_f holds confidence weights (foreground where > 0 and
+ * finite) and conf_w the working voxels' mean foreground weights. */
+void n4o_threshold_log_shrink_w(const float *im, const float *mask_f,
+                                int32_t weighted, const int32_t fdims[3],
+                                double low_value, int32_t s, int32_t wdims[3],
+                                double *imlog_w, uint8_t *mask_w,
+                                double *conf_w, int64_t *n_mask_full,
+                                int64_t *n_mask_work) {
     const int32_t FX = fdims[0], FY = fdims[1], FZ = fdims[2];
     for (int d = 0; d < 3; ++d) wdims[d] = (fdims[d] + s - 1) / s;
     int64_t nfull = 0, nwork = 0;
@@ -473,28 +492,44 @@ void n4o_threshold_log_shrink(const float *im, const float *mask_f,
         for (int32_t J = 0; J < wdims[1]; ++J)
             for (int32_t K = 0; K < wdims[2]; ++K) {
                 int64_t nm = 0, nb = 0;
-                double acc = 0.0;
+                double acc = 0.0, wsum = 0.0;
                 for (int32_t i = I * s; i < (I + 1) * s && i < FX; ++i)
                     for (int32_t j = J * s; j < (J + 1) * s && j < FY; ++j)
                         for (int32_t k = K * s; k < (K + 1) * s && k < FZ; ++k) {
                             int64_t v = ((int64_t)i * FY + j) * FZ + k;
                             ++nb;
                             float x = im[v];
-                            int m = (mask_f ? (mask_f[v] != 0.0f) : 1) &&
-                                    ((double)x > low_value) && isfinite(x);
+                            int in = 1;
+                            if (mask_f && weighted)
+                                in = mask_f[v] > 0.0f && isfinite(mask_f[v]);
+                            else if (mask_f)
+                                in = mask_f[v] != 0.0f;
+                            int m = in && ((double)x > low_value) && isfinite(x);
                             if (!m) continue;
                             ++nm;
                             acc += log((double)x);
+                            wsum += (mask_f && weighted) ? (double)mask_f[v] : 1.0;
                         }
                 int64_t w = ((int64_t)I * wdims[1] + J) * wdims[2] + K;
                 nfull += nm;
                 int fg = (nm > 0) && (2 * nm >= nb);
                 mask_w[w] = (uint8_t)fg;
                 imlog_w[w] = nm > 0 ? acc / (double)nm : 0.0;
+                if (conf_w) conf_w[w] = fg ? wsum / (double)nm : 0.0;
                 nwork += fg;
             }
     *n_mask_full = nfull;
     *n_mask_work = nwork;
+}
+
+void n4o_threshold_log_shrink(const float *im, const float *mask_f,
+                              const int32_t fdims[3], double low_value,
+                              int32_t s, int32_t wdims[3], double *imlog_w,
+                              uint8_t *mask_w, int64_t *n_mask_full,
+                              int64_t *n_mask_work) {
+    n4o_threshold_log_shrink_w(im, mask_f, 0, fdims, low_value, s, wdims,
+                               imlog_w, mask_w, NULL, n_mask_full,
+                               n_mask_work);
 }
 
 /* N4.py:276-332 */
@@ -514,8 +549,11 @@ int32_t n4o_n4(const float *im, const float *mask_f, const int32_t fdims[3],
     const int64_t nw = (int64_t)W[0] * W[1] * W[2];
     double *imlog = malloc(sizeof(double) * nw);
     uint8_t *mask = malloc(nw);
-    n4o_threshold_log_shrink(im, mask_f, fdims, p->low_value, p->shrink, W,
-                             imlog, mask, &r.n_mask_full, &r.n_mask_work);
+    const int weighted = p->weighted && mask_f;
+    double *conf = weighted ? malloc(sizeof(double) * nw) : NULL;
+    n4o_threshold_log_shrink_w(im, mask_f, weighted, fdims, p->low_value,
+                               p->shrink, W, imlog, mask, conf,
+                               &r.n_mask_full, &r.n_mask_work);
 
     double *logU = malloc(sizeof(double) * nw);
     double *field = calloc((size_t)nw, sizeof(double));
@@ -539,7 +577,7 @@ int32_t n4o_n4(const float *im, const float *mask_f, const int32_t fdims[3],
             n4o_sharpen(logU, mask, nw, p->nbins, p->wiener_noise, p->fwhm,
                         sharp, NULL, NULL, NULL);
             for (int64_t v = 0; v < nw; ++v) resid[v] = logU[v] - sharp[v];
-            n4o_fit_bspline3d(resid, mask, W, nc, p->fit_eps, p->nthreads, lres);
+            fit_bspline3d_w(resid, mask, conf, W, nc, p->fit_eps, p->nthreads, lres);
             for (int64_t l = 0; l < L; ++l) lat[l] += lres[l];
             n4o_evaluate_bspline3d(lat, nc, W, wdom, p->nthreads, newf);
             cv = n4o_convergence(field, newf, mask, nw);
@@ -587,6 +625,7 @@ int32_t n4o_n4(const float *im, const float *mask_f, const int32_t fdims[3],
     free(lb);
     free(imlog);
     free(mask);
+    free(conf);
     free(logU);
     free(field);
     free(newf);

@@ -20,6 +20,14 @@
  *     exactly to N4.py.
  *   - A constant masked image (max == min) makes N4.py raise; here the
  *     sharpening step returns the input unchanged (zero residual).
+ *   - Confidence weights (n4o_params.weighted; the reference takes binary
+ *     masks only, cudaN4.h:346): foreground where the weight is > 0 and
+ *     finite; each foreground voxel's numerator and denominator terms of the
+ *     fit (N4.py:230-236) are multiplied by its weight, the way ITK/ANTs N4
+ *     passes a confidence image to its B-spline fit; histogram and
+ *     convergence measure stay unweighted.  With a shrink, a working voxel's
+ *     weight is the mean weight of its block's foreground voxels.  Weights
+ *     all 1 reduce exactly to the binary case.
  *
  * Layout: C order [x][y][z], z fastest (cudaN4.cu:476,562).
  */
@@ -46,6 +54,7 @@ typedef struct n4o_params {
     int32_t shrink;                /* build-defined (1)     */
     double fit_eps;                /* N4.py:186     (1e-3)  */
     int32_t nthreads;              /* OpenMP threads for fit/evaluate (1) */
+    int32_t weighted;              /* mask_f holds confidence weights (0)  */
 } n4o_params;
 
 typedef struct n4o_report {
@@ -104,6 +113,12 @@ void n4o_fit_numden(const double *field, const uint8_t *mask,
                     const int32_t dims[3], const int32_t ncpt[3],
                     int32_t nthreads, double *num, double *den);
 
+/* The same with per-voxel confidence weights conf (nullable: all 1). */
+void n4o_fit_numden_w(const double *field, const uint8_t *mask,
+                      const double *conf, const int32_t dims[3],
+                      const int32_t ncpt[3], int32_t nthreads, double *num,
+                      double *den);
+
 /* EvaluateBspline3D, N4.py:241-273, generalised to a domain length per axis
  * (domain[d] == dims[d] reproduces N4.py). */
 void n4o_evaluate_bspline3d(const double *lattice, const int32_t ncpt[3],
@@ -125,6 +140,16 @@ void n4o_threshold_log_shrink(const float *im, const float *mask_f,
                               int32_t shrink, int32_t wdims[3],
                               double *imlog_w, uint8_t *mask_w,
                               int64_t *n_mask_full, int64_t *n_mask_work);
+
+/* The same with mask_f read as confidence weights when weighted != 0;
+ * conf_w (nullable) receives each working voxel's mean foreground weight
+ * (0 for background working voxels; 1 where weighted == 0). */
+void n4o_threshold_log_shrink_w(const float *im, const float *mask_f,
+                                int32_t weighted, const int32_t fdims[3],
+                                double low_value, int32_t shrink,
+                                int32_t wdims[3], double *imlog_w,
+                                uint8_t *mask_w, double *conf_w,
+                                int64_t *n_mask_full, int64_t *n_mask_work);
 
 /* Whole N4, N4.py:276-332 with the shrink extension.
  *   im, mask_f: full resolution float32, [x][y][z]. mask_f nullable.

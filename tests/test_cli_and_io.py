@@ -71,3 +71,48 @@ def test_cli_emulated_matches_oracle(tmp_path):
 @pytest.mark.gpu
 def test_cli_gpu_matches_oracle(tmp_path):
     _cli_matches_oracle("CUDAN4", tmp_path, (48, 44, 40), PC.LOGBIAS_TOL)
+
+
+def _cli_extensions(exe, tmp_path, dims, tol):
+    """-c i,j,k (file axis order: library x = k), --fixed-iterations,
+    --weighted-mask, --level-lattices and the report's convergence trace,
+    each against the oracle run the same way."""
+    from test_weighted_mask import weights_for
+
+    im, m, _ = PC.small_phantom(dims, seed=6)
+    w = weights_for(m, 7)
+    pi, pm = str(tmp_path / "im.nii.gz"), str(tmp_path / "w.nii.gz")
+    nifti.write(pi, im)
+    nifti.write(pm, w)
+    po, pb, pr, pre = (str(tmp_path / n) for n in ("norm.nii", "bias.nii", "rep.json", "lvl"))
+    r = run_cli(exe, ["-i", pi, "-m", pm, "-o", po, "-b", pb, "-t", "2", "--iters", "3,4", "-c", "6,5,4",
+                      "--threshold", "0.5", "--fixed-iterations", "--weighted-mask", "--level-lattices", pre,
+                      "--report", pr])
+    assert r.returncode == 0, r.stderr
+    ro = O.n4(im, w, levels=2, max_iter=[3, 4], ncpt=(4, 5, 6), conv_threshold=0.0, weighted=1)
+    rep = json.load(open(pr))
+    assert rep["iterations"] == [3, 4] == ro["iterations"]
+    assert rep["ncpt"] == [4, 5, 6] and rep["fixed_iterations"] and rep["weighted_mask"]
+    trace = np.concatenate([np.array(t) for t in rep["convergence_trace"]])
+    assert [len(t) for t in rep["convergence_trace"]] == [3, 4]
+    assert np.abs(trace - ro["cv_trace"]).max() <= 1e-6
+    mm = (w > 0) & (im > 10)
+    assert np.abs(np.log(nifti.read(pb)[mm]) - ro["log_bias"][mm]).max() <= tol
+    l0 = nifti.read(pre + "0.nii.gz")
+    assert l0.shape == (4, 5, 6)
+    r0 = O.n4(im, w, levels=1, max_iter=[3], ncpt=(4, 5, 6), conv_threshold=0.0, weighted=1)
+    assert np.abs(l0 - r0["lattice"]).max() <= 10 * tol
+    l1 = nifti.read(pre + "1.nii.gz")
+    assert l1.shape == (5, 7, 9) and np.abs(l1 - ro["lattice"]).max() <= 10 * tol
+    # -c with two values is refused
+    r = run_cli(exe, ["-i", pi, "-o", po, "-c", "4,5"])
+    assert r.returncode == 255 and "i,j,k" in r.stderr
+
+
+def test_cli_emulated_extensions(tmp_path):
+    _cli_extensions("CUDAN4_emu", tmp_path, (26, 24, 22), PC.LOGBIAS_TOL)
+
+
+@pytest.mark.gpu
+def test_cli_gpu_extensions(tmp_path):
+    _cli_extensions("CUDAN4", tmp_path, (40, 36, 32), PC.LOGBIAS_TOL)
