@@ -250,6 +250,20 @@ static int padded_size(int nbins, int* log2P) {  // N4.py:31-37
     return 1 << (e + 1);
 }
 
+static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
+
+// The MFMA kernels hold up to 16 * kMfmaMaxTiles control points along y and
+// z and take binary masks; a call whose final lattice has more, or that
+// passes confidence weights, runs the FP32 kernels instead (in every level:
+// the launches of one step mix volumes at different levels).
+static bool mfma_covers(const n4_params* p) {
+    if (p->flags & N4_FLAG_WEIGHTED_MASK) return false;
+    const int L = (int)p->number_of_fitting_levels;
+    for (int d = 1; d < 3; ++d)
+        if (level_ncpt((int)p->ncpt[d], L - 1) > 16 * kMfmaMaxTiles) return false;
+    return true;
+}
+
 static RunParams make_run_params(const n4_params* p) {
     RunParams rp;
     memset(&rp, 0, sizeof rp);
@@ -265,7 +279,7 @@ static RunParams make_run_params(const n4_params* p) {
     if ((double)f > p->low_value) f = nextafterf(f, -INFINITY);
     rp.low_f = f;
     rp.fixed_iters = (p->flags & N4_FLAG_FIXED_ITERATIONS) ? 1 : 0;
-    rp.prec = (int)p->precision;
+    rp.prec = mfma_covers(p) ? (int)p->precision : 0;
     return rp;
 }
 
@@ -315,7 +329,6 @@ struct Plan {
     unsigned char* base = nullptr;  // device arena the plan lives in
 };
 
-static int level_ncpt(int c0, int level) { return (c0 - 3) * (1 << level) + 3; }
 
 // Row tiles of the step kernels on a working grid W: rows per tile (whole
 // multiples of 16 rows, one MFMA row tile per wave) and tiles per slab.
@@ -740,13 +753,6 @@ static n4_status check_params(const n4_params* p) {
         return fail(N4_ERR_UNSUPPORTED, "precision %u: the host emulation build runs the FP32 kernels only",
                     p->precision);
 #endif
-    if (p->precision != N4_PREC_FP32)
-        for (int d = 1; d < 3; ++d) {
-            const uint64_t cf = ((uint64_t)(p->ncpt[d] - 3) << (p->number_of_fitting_levels - 1)) + 3;
-            if (cf > (uint64_t)(16 * kMfmaMaxTiles))
-                return fail(N4_ERR_UNSUPPORTED, "MFMA precision modes support up to %d control points along y and z "
-                            "(final level has %llu along axis %d)", 16 * kMfmaMaxTiles, (unsigned long long)cf, d);
-        }
     // final lattice size must stay sane
     uint64_t Lf = 1;
     for (int d = 0; d < 3; ++d) {
@@ -971,13 +977,13 @@ static bool plan_teams(const std::vector<VolIn>& in, const n4_params* p, const S
         return false;
     }
 #endif
-    if (p->precision != N4_PREC_FP32)
+    const RunParams rp = make_run_params(p);
+    if (rp.prec != 0)
         for (int d = 1; d < 3; ++d)
             if (level_ncpt((int)p->ncpt[d], L - 1) > 16) {
                 *why = "the MFMA fit of the persistent path covers up to 16 control points along y and z";
                 return false;
             }
-    const RunParams rp = make_run_params(p);
     size_t lds = 0;
     std::vector<int> tiles(nv);
     std::vector<double> w(nv);

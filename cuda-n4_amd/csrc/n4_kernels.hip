@@ -352,10 +352,15 @@ template <int S, int MK>
 HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, int& nm, int& nb, float& acc,
                    float& wsum) {
     const int F0 = V.F[0], F1 = V.F[1], F2 = V.F[2];
-    const int ss = S > 0 ? S : V.s;
+    // Unrolled interior blocks for shrink 3 and 4 (shrink 1 and 2 fold their
+    // interior blocks in body_prep_t and come here for the border only).
+    // Confidence masks take the generic loop with a run-time block size
+    // (their float mask values would raise every shrink class's registers).
+    constexpr bool kUnrolled = S > 2 && MK != 3;
+    const int ss = (S > 0 && MK != 3) ? S : V.s;
     const int i0 = x * ss, j0 = J * ss, k0 = K * ss;
     const bool interior = (i0 + ss <= F0) && (j0 + ss <= F1) && (k0 + ss <= F2);
-    if (S > 0 && interior) {
+    if (kUnrolled && interior) {
         constexpr int SS = S > 0 ? S : 1;
         UNROLL for (int a = 0; a < SS; ++a) {
             float xv[SS * SS];
@@ -367,9 +372,14 @@ HD void prep_voxel(const VolDesc& V, const RunParams& rp, int x, int J, int K, i
                     mv[b * SS + c] = prep_mask<MK>(V, fv);
                 }
             }
+            // (written out rather than through prep_fold: this form keeps
+            // the shrink-3 and -4 kernels at 49 and 64 VGPRs)
             UNROLL for (int q = 0; q < SS * SS; ++q) {
                 ++nb;
-                prep_fold<MK>(mv[q], xv[q], rp, nm, acc, wsum);
+                if (mv[q] && xv[q] > rp.low_f && isfinite(xv[q])) {
+                    ++nm;
+                    acc += logf(xv[q]);
+                }
             }
         }
         return;
@@ -419,7 +429,7 @@ HD void body_prep_t(const VolDesc& V, const Slab sl, const RunParams& rp, N4_LDS
     // prep_voxel).  Larger shrinks keep one working voxel per iteration
     // (their blocks already hold 27+ loads; the shrink cases share one
     // kernel, whose register count is that of its largest case).
-    constexpr bool kBatched = S >= 1 && S <= 2;
+    constexpr bool kBatched = S >= 1 && S <= 2 && MK != 3;
     BLOCK_FOR(t)
     double nfull = 0.0, nwork = 0.0;
     double mn = INFINITY, mx = -INFINITY;

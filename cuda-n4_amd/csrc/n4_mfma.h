@@ -244,9 +244,8 @@ __device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab 
         UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
         for (int zc = 0; zc < Z; zc += 64) {
             if (rb > 0 || zc > 0) fit_mfma_load(V, sl, y0, zc, den, ms, xs);  // (the first block is already in registers)
-            const size_t vrow = ((size_t)x * V.W[1] + sl.y0 + y0) * Z + zc + lane;  // row y0, this lane's z
             UNROLL for (int r = 0; r < 16; ++r) {
-                const float gv = conf_scale(V, ms[r], vrow + (size_t)r * Z, fit_input(gmode, ms[r], xs[r], degenerate, rm));
+                const float gv = fit_input(gmode, ms[r], xs[r], degenerate, rm);
                 unsigned short h, l = 0;
                 if constexpr (PREC == 2)
                     bf16_split(gv, h, l);

@@ -51,7 +51,8 @@ typedef enum n4_precision {
  * scaled by the weight, as ITK/ANTs N4 uses a confidence image; histogram
  * and convergence measure stay unweighted over the foreground.  With a
  * shrink factor the working voxel's weight is the mean weight of the
- * foreground voxels of its block.  U8 and absent masks stay binary. */
+ * foreground voxels of its block.  U8 and absent masks stay binary.  The
+ * fit and evaluation then run the FP32 kernels whatever the precision. */
 #define N4_FLAG_WEIGHTED_MASK 0x2u
 
 /* Parameters.  Replaces struct N4Param (cudaN4.h:24-94); N4.py:276-279,

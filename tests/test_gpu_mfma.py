@@ -99,11 +99,17 @@ def test_bf16_deterministic(gpu):
 
 
 def test_precision_limits(gpu):
-    """The MFMA kernels cover up to 64 control points along y and z."""
+    """The MFMA kernels cover up to 64 control points along y and z; a call
+    whose final lattice has more (ncpt 5 at 6 levels: 67 along z) runs the
+    FP32 kernels, bit for bit the FP32 call.  An unknown precision value is
+    refused."""
     import n4amd
     im, m, _ = PC.small_phantom()
-    with pytest.raises(n4amd.N4Error):
-        gpu.correct(im, m, levels=6, ncpt=(4, 4, 5), precision="bf16")  # 5 -> 67 along z
+    kw = dict(levels=6, ncpt=(4, 4, 5), max_iter=3, fixed_iterations=True)
+    a = gpu.correct(im, m, precision="bf16", **kw)
+    b = gpu.correct(im, m, precision="fp32", **kw)
+    assert a["lattice"].shape == (35, 35, 67)
+    assert np.array_equal(a["log_bias"], b["log_bias"])
     with pytest.raises(n4amd.N4Error):
         gpu.correct(im, m, precision=7)
 
