@@ -1217,11 +1217,6 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
         GLAUNCH(N4_K_TEAMS, launch_persist(dd, wsl, (const PWork*)(P.base + P.off_work), P.n_work,
                                            rp, P.lds_persist, gs));
         GLAUNCH(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, gs));
-        for (int v = 0; v < nv; ++v)
-            if (in[v].out_lattice) {
-                const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
-                D2D(in[v].out_lattice, P.desc[v].latf, sizeof(float) * LD.L, gs);
-            }
         for (GroupRun& R : gr) R.finished = true;
     }
     int remaining = teams_path ? 0 : G;
@@ -1283,12 +1278,8 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                 R.it += chunk;
                 continue;
             }
+            // (the final step wrote each volume's output lattice)
             GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon, gs));
-            for (int v = R.v0; v < R.v1; ++v)
-                if (in[v].out_lattice) {
-                    const LevelDesc& LD = P.desc[v].lev[P.nlevels - 1];
-                    D2D(in[v].out_lattice, P.desc[v].latf, sizeof(float) * LD.L, gs);
-                }
 #ifndef N4_EMULATE
             if (g > 0) {
                 N4_CHECK_HIP(hipEventRecord(h->join_ev[g], (hipStream_t)gs));

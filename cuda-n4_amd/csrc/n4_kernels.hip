@@ -1035,7 +1035,8 @@ HD void wiener_table(const VolDesc& V, double bmin, double slope, RunParams rp, 
 // A volume's per-step work and the state for the next step (one block):
 //   ITER   Wiener step (unless degenerate); next: ITER(l, it+1)
 //   TRANS  refine the lattice into level l (l > 0); next: ITER(l, 0)
-//   FINAL  FP32 copy of the final lattice; next: finished
+//   FINAL  FP32 copy of the final lattice (and the caller's output lattice);
+//          next: finished
 // WM: where the histogram copies come from (WienerMode).
 template <int WM>
 HD void volume_step(const VolDesc& V, int kind, int s_lvl, int s_it, bool degenerate, double bmin, double slope,
@@ -1066,8 +1067,10 @@ HD void volume_step(const VolDesc& V, int kind, int s_lvl, int s_it, bool degene
         float* rec = level_lattice_record(V, V.nlevels - 1);
         BLOCK_FOR(t)
         for (int l = t; l < FD.L; l += kBlock) {
-            V.latf[l] = (float)fin[l];
-            if (rec) rec[l] = (float)fin[l];
+            const float f = (float)fin[l];
+            V.latf[l] = f;
+            if (rec) rec[l] = f;
+            if (V.out_lattice) V.out_lattice[l] = f;
         }
         if (t == 0) {
             st->mode = kModeFinished;
@@ -2269,6 +2272,7 @@ HD void body_persist(const VolDesc* vols, const Slab* slabs, const PWork* work, 
                     const float f = (float)ld_agent(fin + l);
                     V.latf[l] = f;
                     if (rec) rec[l] = f;
+                    if (V.out_lattice) V.out_lattice[l] = f;
                 }
                 if (t == 0) {
                     st->mode = kModeFinished;
