@@ -280,6 +280,8 @@ static RunParams make_run_params(const n4_params* p) {
     rp.low_f = f;
     rp.fixed_iters = (p->flags & N4_FLAG_FIXED_ITERATIONS) ? 1 : 0;
     rp.prec = mfma_covers(p) ? (int)p->precision : 0;
+    rp.nlevels = (int)p->number_of_fitting_levels;
+    for (int l = 0; l < kMaxLevels; ++l) rp.max_iter[l] = (int)p->max_iterations[l];
     return rp;
 }
 
@@ -569,6 +571,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         base = (unsigned char*)h->arena;
     }
     P->base = base;
+    P->rp.states = (VolState*)(base + P->off_state);
+    P->rp.descs = (const VolDesc*)(base + P->off_desc);
 
     // ---- host image of the upload region
     h->staging.assign(P->upload_bytes, 0);

@@ -153,6 +153,17 @@ struct RunParams {
     float low_f;      // largest float <= low: x > low_f  <=>  (double)x > low
     int32_t fixed_iters;
     int32_t prec;     // n4_precision: 0 FP32, 1 BF16 MFMA, 2 split-BF16 MFMA
+    // Call-wide values the step kernels would otherwise load through the
+    // volume descriptor after the state (kernel arguments arrive in scalar
+    // registers at launch): the volumes' states (volume v at states + v,
+    // the same address as its VolDesc::st), the level count and the
+    // per-level iteration limits.  Kernels launched over a group's volumes
+    // (a sub-range of the descriptor array `descs`) find a volume's state at
+    // states + (its descriptor - descs).
+    VolState* states;
+    const VolDesc* descs;
+    int32_t nlevels;
+    int32_t max_iter[kMaxLevels];
 };
 
 // Persistent path: one workgroup of a volume's team.  The team's

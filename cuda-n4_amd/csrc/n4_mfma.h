@@ -312,7 +312,7 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
                               int bid, N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    const VolState* st = V.st;
+    const VolState* st = rp.states + sl.vol;
     if (!fit_step_mode(st, &mode, &level, &it)) return;
     const LevelDesc& LD = V.lev[level];
     const int Z = V.W[2];
@@ -444,8 +444,9 @@ template <int PREC, int MT, int NT>
 __device__ void body_eval_mfma(const VolDesc* vols, const Slab* slabs, RunParams rp, int bid, N4_LDS unsigned char* smem) {
     const Slab sl = slabs[bid];
     const VolDesc& V = vols[sl.vol];
-    if (V.st->s_kind != kStepIter) return;
-    const int level = V.st->s_lvl;
+    const VolState* st = rp.states + sl.vol;
+    if (st->s_kind != kStepIter) return;
+    const int level = st->s_lvl;
     const LevelDesc& LD = V.lev[level];
     const int Y = V.W[1], Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
