@@ -288,12 +288,19 @@ def main():
     lib = n4amd.load()
     log(f"library {lib.version()} ({lib.path})")
     if args.copy_calibration_mib:
+        # counters calibrated on kernels of known traffic: a device copy
+        # (nbytes read + nbytes written) and the library's read-only sum
+        # kernel (k_flat: nbytes read, a few KB of partials written)
         nbytes = args.copy_calibration_mib << 20
         a, b = lib.empty(nbytes), lib.empty(nbytes)
+        lib._check(lib.L.n4_memset(lib.h, a.ptr, 0, nbytes))
         for _ in range(3):
             lib._check(lib.L.n4_memcpy_dtod(lib.h, b.ptr, a.ptr, nbytes))
+        s = C.c_double()
+        for _ in range(3):
+            lib._check(lib.L.n4_reduce_sum(lib.h, nbytes // 4, a.ptr, C.byref(s)))
         lib.synchronize()
-        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3, "read_only_sums": 3}), flush=True)
         return
 
     base_kw = dict(levels=cfg["levels"], max_iter=cfg["max_iterations"], threshold=0.001, shrink=shrinks[0])

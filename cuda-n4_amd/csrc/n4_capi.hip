@@ -914,6 +914,10 @@ struct GraphKey {
         add(r.low_f);
         add(r.fixed_iters);
         add(r.prec);
+        add(r.states);
+        add(r.descs);
+        add(r.nlevels);
+        for (int l = 0; l < kMaxLevels; ++l) add(r.max_iter[l]);
     }
 };
 
@@ -1223,27 +1227,25 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
         GLAUNCH(N4_K_RECON, launch_recon(dd, fsl, P.n_fslab, P.nlevels - 1, rp, P.lds_recon, gs));
         for (GroupRun& R : gr) R.finished = true;
     }
-    int remaining = teams_path ? 0 : G;
-    uint64_t idle_rounds = 0;
-    while (remaining > 0) {
-        bool advanced = false;
-        for (int g = 0; g < G; ++g) {
-            GroupRun& R = gr[g];
-            if (R.finished) continue;
-            void* gs = R.stream;
-            bool end = R.it >= max_steps;
-            if (R.it > 0 && !end && !noalloc) {
-                bool ready = false, all_done = false;
-                s = check_progress(h, R.v0, R.v1, 0, R.it, depth, gs, (idle_rounds & 63) == 63, &ready,
-                                   &all_done);
-                if (s != N4_OK) return s;
-                if (!ready) continue;
-                end = all_done;
-            }
-            advanced = true;
-            const int nvg = R.v1 - R.v0, nws = R.ws1 - R.ws0;
-            const VolDesc* gv = dd + R.v0;
-            const Slab* gsl = wsl + R.ws0;
+    // One group's next move: wait (0), enqueue a chunk of steps (1), or,
+    // once every volume of the group has finished, its reconstruction and
+    // join event (2); < 0: failed (the status is in *err).
+    auto advance_group = [&](int g, bool query, n4_status* err) -> int {
+        GroupRun& R = gr[g];
+        void* gs = R.stream;
+        bool end = R.it >= max_steps;
+        n4_status s2 = N4_OK;
+        if (R.it > 0 && !end && !noalloc) {
+            bool ready = false, all_done = false;
+            s2 = check_progress(h, R.v0, R.v1, 0, R.it, depth, gs, query, &ready, &all_done);
+            if (s2 != N4_OK) return *err = s2, -1;
+            if (!ready) return 0;
+            end = all_done;
+        }
+        const int nvg = R.v1 - R.v0, nws = R.ws1 - R.ws0;
+        const VolDesc* gv = dd + R.v0;
+        const Slab* gsl = wsl + R.ws0;
+        auto body = [&]() -> n4_status {
             if (!end) {
                 auto enqueue = [&](void* gs) -> n4_status {
                     for (int k = 0; k < chunk; ++k) {
@@ -1255,6 +1257,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                     }
                     return N4_OK;
                 };
+                n4_status s3 = N4_OK;
                 if (use_graph) {
 #ifndef N4_EMULATE
                     GraphKey key;
@@ -1273,14 +1276,14 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
                     key.add(upd_chunks);
                     key.add(rp_eval.prec);
                     key.add_rp(rp);
-                    s = launch_step_graph(h, g, key, enqueue, gs);
+                    s3 = launch_step_graph(h, g, key, enqueue, gs);
 #endif
                 } else {
-                    s = enqueue(gs);
+                    s3 = enqueue(gs);
                 }
-                if (s != N4_OK) return s;
+                if (s3 != N4_OK) return s3;
                 R.it += chunk;
-                continue;
+                return N4_OK;
             }
             // (the final step wrote each volume's output lattice)
             GLAUNCH(N4_K_RECON, launch_recon(dd, fsl + R.fs0, R.fs1 - R.fs0, P.nlevels - 1, rp, P.lds_recon, gs));
@@ -1291,7 +1294,26 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             }
 #endif
             R.finished = true;
-            --remaining;
+            return N4_OK;
+        };
+        s2 = body();
+        if (s2 != N4_OK) return *err = s2, -1;
+        return R.finished ? 2 : 1;
+    };
+    // one host thread drives every group, round robin (a thread per group
+    // measured no faster: the host's launch cost is not the bound)
+    int remaining = teams_path ? 0 : G;
+    uint64_t idle_rounds = 0;
+    while (remaining > 0) {
+        bool advanced = false;
+        for (int g = 0; g < G; ++g) {
+            if (gr[g].finished) continue;
+            n4_status e = N4_OK;
+            const int r = advance_group(g, (idle_rounds & 63) == 63, &e);
+            if (r < 0) return e;
+            if (r == 0) continue;
+            advanced = true;
+            if (r == 2) --remaining;
         }
         if (advanced) {
             idle_rounds = 0;

@@ -1761,8 +1761,9 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
         N4_GLOBAL float* o_b = gp(V.out_bias);
         N4_GLOBAL float* o_n = gp(V.out_norm);
         // consecutive lanes on consecutive voxels, kReconLoads image loads
-        // in flight per thread
-        constexpr int kReconLoads = 16;
+        // in flight per thread (8 measured 9% faster than 16 once the
+        // division became a product)
+        constexpr int kReconLoads = 8;
         for (int v0 = 0; v0 < ny * Z; v0 += kReconLoads * kBlock) {
             float xin[kReconLoads];
             UNROLL for (int u = 0; u < kReconLoads; ++u) {
@@ -1777,13 +1778,12 @@ HD void body_recon(const VolDesc* vols, const Slab* slabs, int level, RunParams 
                 const N4_LDS float* br = S.B + r * Cz + S.zb[z];
                 const float f = wz[0] * br[0] + wz[1] * br[1] + wz[2] * br[2] + wz[3] * br[3];
                 if (o_lb) o_lb[base + v] = f;
-                if (o_n || o_b) {
-                    const float ef = expf(f);
-                    if (o_b) o_b[base + v] = ef;
-                    if (o_n) {
-                        const float xo = (xin[u] <= rp.low_f) ? 0.0f : xin[u];  // N4.py:285
-                        o_n[base + v] = xo / ef;
-                    }
+                if (o_b) o_b[base + v] = expf(f);
+                if (o_n) {
+                    // thresholded image / exp(f) as a product with exp(-f):
+                    // no IEEE division per voxel (within 2 ulp of the quotient)
+                    const float xo = (xin[u] <= rp.low_f) ? 0.0f : xin[u];  // N4.py:285
+                    o_n[base + v] = xo * expf(-f);
                 }
             }
         }

@@ -59,6 +59,27 @@ def test_graph_recaptured_when_inputs_change(gpu):
         gpu.set_tuning(reset=True)
 
 
+def test_graph_recaptured_when_iteration_limits_change(gpu):
+    """The per-level iteration limits travel in the launches' arguments: a
+    step graph captured for one set of limits must not replay for another
+    (same volumes, pointers and sizes otherwise)."""
+    vols = _batch()[:2]
+    lims = ([3, 3], [5, 2])
+    kw = dict(levels=2, fixed_iterations=True)
+    try:
+        gpu.set_tuning(graph_steps="off")
+        ref = [gpu.correct_batch(vols, max_iter=m, **kw) for m in lims]
+        gpu.set_tuning(graph_steps=4)
+        for _ in range(2):
+            for m, r in zip(lims, ref):
+                out = gpu.correct_batch(vols, max_iter=m, **kw)
+                for a, b in zip(r, out):
+                    assert a["iterations"] == b["iterations"] == m
+                    assert np.array_equal(a["log_bias"], b["log_bias"])
+    finally:
+        gpu.set_tuning(reset=True)
+
+
 def test_single_volume_graph_parity(gpu):
     """The drop-in single-volume call with step graphs keeps the oracle
     parity (C0-size case with early exit)."""
