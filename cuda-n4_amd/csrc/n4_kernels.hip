@@ -121,6 +121,17 @@ HD F4 load_quad(const float* p) {
 #endif
 }
 
+// One 16-byte global store of a voxel quad (p 16-byte aligned).
+HD void store_quad(float* p, const F4& x) {
+#ifdef N4_DEVICE_PASS
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    v4f y = {x.v[0], x.v[1], x.v[2], x.v[3]};
+    *reinterpret_cast<N4_GLOBAL v4f*>(gp(p)) = y;
+#else
+    *reinterpret_cast<F4*>(p) = x;
+#endif
+}
+
 // Data shared between the workgroups of a persistent team (partials,
 // histogram and numerator copies, lattice, increments, barrier counter) is
 // written only with agent-scope atomics / stores and read only with
@@ -1580,33 +1591,49 @@ HD void eval_chunks(const VolDesc& V, const LevelDesc& LD, const Slab sl, const 
         double s1 = S.red[t], s2 = S.red[kBlock + t];
         // min/max of FP32 values are exact in FP32
         float mn = (float)S.red[2 * kBlock + t], mx = (float)S.red[3 * kBlock + t];
-        // consecutive lanes on consecutive voxels (conflict-free LDS
-        // reads of the z tables), kEvalLoads mask and logU loads in flight
-        // per thread, all independent (logU is read whatever the mask)
-        constexpr int kEvalLoads = 8;
-        for (int v0 = 0; v0 < ny * Z; v0 += kEvalLoads * kBlock) {
-            uint8_t ms[kEvalLoads];
-            float lu[kEvalLoads];
-            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
-                const int v = v0 + u * kBlock + t;
-                const bool ok = v < ny * Z;
-                ms[u] = ok ? gp(V.wmask)[base + v] : 0;
-                lu[u] = ok ? gp(V.logU)[base + v] : 0.0f;
+        // aligned voxel quads (one 16-byte logU load and one 4-byte mask
+        // load per 4 voxels: a quarter of the memory instructions of
+        // per-voxel accesses), kEvalQuads quads in flight per thread, logU
+        // read whatever the mask; a quad wholly inside the chunk is stored
+        // with one 16-byte store, a quad straddling its ends voxel by voxel
+        // (the voxels outside belong to other tiles)
+        constexpr int kEvalQuads = 2;
+        const size_t vend = base + (size_t)ny * Z;
+        const size_t q0 = base & ~(size_t)3;
+        const int nqd = (int)((vend - q0 + 3) >> 2);
+        for (int i0 = 0; i0 < nqd; i0 += kEvalQuads * kBlock) {
+            F4 xs[kEvalQuads];
+            uint32_t ms[kEvalQuads];
+            UNROLL for (int u = 0; u < kEvalQuads; ++u) {
+                const int qi = i0 + u * kBlock + t;
+                const size_t w = q0 + 4 * (size_t)qi;
+                const bool ok = qi < nqd;
+                ms[u] = ok ? *reinterpret_cast<const N4_GLOBAL uint32_t*>(gp(V.wmask) + w) : 0u;
+                xs[u] = ok ? load_quad(V.logU + w) : F4{{0.0f, 0.0f, 0.0f, 0.0f}};
             }
-            UNROLL for (int u = 0; u < kEvalLoads; ++u) {
+            UNROLL for (int u = 0; u < kEvalQuads; ++u) {
                 if (!ms[u]) continue;
-                const int v = v0 + u * kBlock + t;
-                const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
-                const N4_LDS float* wz = S.zw + 4 * z;
-                const N4_LDS float* brr = S.Br + r * Cz + S.zb[z];
-                const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
-                const float lg = lu[u] - d;
-                gp(V.logU)[base + v] = lg;
-                const double uu = (double)expm1f(-d);
-                s1 += uu;
-                s2 += uu * uu;
-                mn = fminf(mn, lg);
-                mx = fmaxf(mx, lg);
+                const size_t wq = q0 + 4 * (size_t)(i0 + u * kBlock + t);
+                const bool full = wq >= base && wq + 4 <= vend;
+                F4 o = xs[u];
+                UNROLL for (int j = 0; j < 4; ++j) {
+                    const size_t w = wq + j;
+                    if (w < base || w >= vend || !((ms[u] >> (8 * j)) & 0xffu)) continue;
+                    const int v = (int)(w - base);
+                    const int r = (int)fdiv((uint32_t)v, V.mW2), z = v - r * Z;
+                    const N4_LDS float* wz = S.zw + 4 * z;
+                    const N4_LDS float* brr = S.Br + r * Cz + S.zb[z];
+                    const float d = wz[0] * brr[0] + wz[1] * brr[1] + wz[2] * brr[2] + wz[3] * brr[3];
+                    const float lg = xs[u].v[j] - d;
+                    o.v[j] = lg;
+                    if (!full) gp(V.logU)[w] = lg;
+                    const double uu = (double)expm1f(-d);
+                    s1 += uu;
+                    s2 += uu * uu;
+                    mn = fminf(mn, lg);
+                    mx = fmaxf(mx, lg);
+                }
+                if (full) store_quad(V.logU + wq, o);
             }
         }
         S.red[t] = s1;

@@ -173,35 +173,37 @@ HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, in
 // Thread t's share of the fit's BF16 factor tables: Pz^T over the z extent
 // (NT*16 rows) and Py^T over rows [ylo, ylo + ny) of the y axis (MT*16
 // rows, columns tile-local); kf 2: w^2 (denominator), else w^3/sum w^2.
-// One entry per thread: consecutive threads fill consecutive positions of
-// a control point's row.
+// One axis position per thread (z positions first, then y): its entry is
+// loaded once (one round of independent global loads for the whole
+// block) and the thread fills the position's column, zeros outside the
+// 4-point support and in the padding; consecutive threads write
+// consecutive columns of a control point's row.
 template <int MT, int NT>
 __device__ void fit_mfma_tables_t(const LevelDesc& LD, const FitMfmaLds& S, int Z, int ylo, int ny, int kf, bool pz,
                                   bool py, int t) {
-    const int Cy = LD.C[1], Cz = LD.C[2];
-    const int Zp = round_up(Z, 32), Yp = round_up(ny, 16);
-    if (pz)
-        for (int e = t; e < NT * 16 * Zp; e += kBlock) {
-            const int c = e / Zp, z = e - c * Zp;
-            float v = 0.0f;
-            if (z < Z && c < Cz) {
-                const AxisEntry& a = LD.ax[2][z];
-                const int k = c - a.b;
-                if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
-            }
-            S.Pz.put(c, z, v);
+    const int Zp = pz ? round_up(Z, 32) : 0, Yp = py ? round_up(ny, 16) : 0;
+    for (int e = t; e < Zp + Yp; e += kBlock) {
+        const bool isz = e < Zp;
+        const int pos = isz ? e : e - Zp;
+        const bool live = isz ? pos < Z : pos < ny;
+        const int C = isz ? LD.C[2] : LD.C[1];
+        float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        int b = 0;
+        if (live) {
+            const AxisEntry& a = isz ? LD.ax[2][pos] : LD.ax[1][ylo + pos];
+            b = a.b;
+            UNROLL for (int k = 0; k < 4; ++k) f[k] = kf == 2 ? a.q[k] : a.p[k];
         }
-    if (py)
-        for (int e = t; e < MT * 16 * Yp; e += kBlock) {
-            const int c = e / Yp, y = e - c * Yp;
+        const Tab& T = isz ? S.Pz : S.Py;
+        const int rows = isz ? NT * 16 : MT * 16;
+        for (int c = 0; c < rows; ++c) {
+            const int k = c - b;
             float v = 0.0f;
-            if (y < ny && c < Cy) {
-                const AxisEntry& a = LD.ax[1][ylo + y];
-                const int k = c - a.b;
-                if (k >= 0 && k < 4) v = kf == 2 ? a.q[k] : a.p[k];
-            }
-            S.Py.put(c, y, v);
+            UNROLL for (int a = 0; a < 4; ++a)
+                if (k == a) v = f[a];
+            T.put(c, pos, (live && c < C) ? v : 0.0f);
         }
+    }
 }
 
 // The voxel block (16 tile-local rows from y0, 64 z from zc) of one wave:
