@@ -19,7 +19,7 @@
 namespace n4 {
 
 constexpr int kBlock = 256;
-constexpr int kHistLanes = 16;  // LDS histogram copies per workgroup (power of two)
+constexpr int kHistLanes = 8;   // LDS histogram copies per workgroup (power of two)
 constexpr int kMaxCopies = 16;  // most global copies of a volume's histogram / fit numerator
 // logU and the working mask are allocated with kQuadPad extra elements, so
 // 16-byte (4-voxel) loads of aligned quads may run past a tile's end.
