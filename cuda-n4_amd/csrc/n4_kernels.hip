@@ -2158,7 +2158,7 @@ HD size_t carve_persist(N4_LDS unsigned char* base, const VolDesc& V, int nbins,
         L.M.Py.has_lo = true;
         L.M.Py.hi = f.take<unsigned short>((size_t)16 * ys);
         L.M.Py.lo = f.take<unsigned short>((size_t)16 * ys);
-        L.M.red = f.take<float>(256);
+        L.M.red = f.take<float>(4 * 256);  // a slot per wave (fit_mfma_tile<PREC, 1, 1>)
         L.M.Gh = f.take<unsigned short>((size_t)4 * 16 * kGStride);
         L.M.Gl = prec == 2 ? f.take<unsigned short>((size_t)4 * 16 * kGStride) : nullptr;
         um = f.off > um ? f.off : um;

@@ -162,7 +162,9 @@ HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, in
     L.Py.has_lo = true;
     L.Py.hi = c.take<unsigned short>((size_t)MT * 16 * ys);
     L.Py.lo = c.take<unsigned short>((size_t)MT * 16 * ys);
-    L.red = c.take<float>((size_t)MT * NT * 256);
+    // one slot per wave for up to two tiles of control points (a single
+    // barrier before the combine), else one running slot
+    L.red = c.take<float>((size_t)MT * NT * 256 * (MT * NT <= 2 ? 4 : 1));
     L.Gh = c.take<unsigned short>((size_t)4 * 16 * kGStride);
     L.Gl = prec == 2 ? c.take<unsigned short>((size_t)4 * 16 * kGStride) : nullptr;
     L.ex = c.take<AxisEntry>(1);
@@ -284,27 +286,38 @@ __device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab 
             }
         }
     }
-    // waves combine in a fixed order (bitwise reproducible)
-    for (int w = 0; w < 4; ++w) {
-        if (wv == w) {
-            UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) UNROLL for (int i = 0; i < 4; ++i) {
-                const int idx = (((m * NT + n) * 4 + i) << 6) + lane;
-                S.red[idx] = (w == 0 ? 0.0f : S.red[idx]) + acc[m][n][i];
-            }
-        }
+    // waves combine in a fixed order (bitwise reproducible): with a slot
+    // per wave, one barrier and the sum over the slots in wave order;
+    // else the waves add into one slot in turn
+    constexpr int kE = MT * NT * 256;
+    constexpr bool kSlots = MT * NT <= 2;
+    if constexpr (kSlots) {
+        UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) UNROLL for (int i = 0; i < 4; ++i)
+            S.red[wv * kE + (((m * NT + n) * 4 + i) << 6) + lane] = acc[m][n][i];
         __syncthreads();
+    } else {
+        for (int w = 0; w < 4; ++w) {
+            if (wv == w) {
+                UNROLL for (int m = 0; m < MT; ++m) UNROLL for (int n = 0; n < NT; ++n) UNROLL for (int i = 0; i < 4; ++i) {
+                    const int idx = (((m * NT + n) * 4 + i) << 6) + lane;
+                    S.red[idx] = (w == 0 ? 0.0f : S.red[idx]) + acc[m][n][i];
+                }
+            }
+            __syncthreads();
+        }
     }
     float* out = V.part + (size_t)x * V.part_stride;
     const AxisEntry ex = *S.ex;
-    for (int e = t; e < MT * NT * 256; e += kBlock) {
+    for (int e = t; e < kE; e += kBlock) {
         const int ln = e & 63, i = (e >> 6) & 3, tile = e >> 8;
         const int m = tile / NT, n = tile - (tile / NT) * NT;
         const int cy = m * 16 + 4 * (ln >> 4) + i, cz = n * 16 + (ln & 15);
         if (cy < Cy && cz < Cz) {
+            const float r = kSlots ? ((S.red[e] + S.red[kE + e]) + S.red[2 * kE + e]) + S.red[3 * kE + e] : S.red[e];
             if (scatter)
-                scatter_numerator(V, LD, ex, tile_id(V, sl) % V.num_copies, den, cy * Cz + cz, S.red[e]);
+                scatter_numerator(V, LD, ex, tile_id(V, sl) % V.num_copies, den, cy * Cz + cz, r);
             else
-                out[cy * Cz + cz] = S.red[e];
+                out[cy * Cz + cz] = r;
         }
     }
 }
