@@ -1185,7 +1185,8 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     Plan P;
     std::vector<unsigned char> pkey;
     if (!noalloc) pkey = correction_plan_key(h, in, p, sch.tile_vox, teams_path ? &teams : nullptr);
-    if (!noalloc && h->plan_cached && pkey == h->plan_key) {
+    const bool plan_reused = !noalloc && h->plan_cached && pkey == h->plan_key;
+    if (plan_reused) {
         // the plan of the last call, whose upload image is still in staging:
         // only the caller's buffers change
         P = *h->plan_cached;
@@ -1210,7 +1211,15 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     }
     s = check_lds(P, !teams_path, true);
     if (s != N4_OK) return s;
-    H2D(P.base, h->staging.data(), P.upload_bytes, stream);
+    if (plan_reused) {
+        // the arena still holds the rest of the upload region (slab lists,
+        // twiddles, axis tables; no kernel writes them): descriptors, and the
+        // volume states back to their initial (zero) image
+        H2D(P.base + P.off_desc, h->staging.data() + P.off_desc, sizeof(VolDesc) * nv, stream);
+        H2D(P.base + P.off_state, h->staging.data() + P.off_state, sizeof(VolState) * nv, stream);
+    } else {
+        H2D(P.base, h->staging.data(), P.upload_bytes, stream);
+    }
     VolDesc* dd = dev_desc(h, P);
     Slab* wsl = dev_wslab(h, P);
     Slab* fsl = dev_fslab(h, P);
