@@ -19,6 +19,7 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -601,6 +602,25 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         }
         P->rp.twiddle = (const D2*)(base + P->off_tw);
     }
+    // identical axis tables (same positions, domain and control points) are
+    // computed once per plan and shared by every volume and level using them
+    // (working-grid tables, with their support ranges, and full-resolution
+    // tables, without, are kept apart)
+    std::map<std::tuple<bool, int, double, int>, std::pair<size_t, size_t>> ax_memo;
+    auto axis_table_at = [&](int npos, double domain, int ncpt, size_t oa, size_t os, int32_t* sup_tmp,
+                             size_t* oa_used, size_t* os_used) {
+        const auto key = std::make_tuple(sup_tmp != nullptr, npos, domain, ncpt);
+        const auto it = ax_memo.find(key);
+        if (it != ax_memo.end()) {
+            *oa_used = it->second.first;
+            *os_used = it->second.second;
+            return;
+        }
+        axis_table(npos, domain, ncpt, (AxisEntry*)(hb + oa), sup_tmp ? sup_tmp : (int32_t*)(hb + os));
+        ax_memo.emplace(key, std::make_pair(oa, os));
+        *oa_used = oa;
+        *os_used = os;
+    };
     for (int v = 0; v < nv; ++v) {
         const VInfo& I = vi[v];
         VolDesc D;
@@ -658,9 +678,9 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             LevelDesc& LD = D.lev[l];
             for (int d = 0; d < 3; ++d) {
                 LD.C[d] = I.C[l][d];
-                const size_t oa = off_ax[(v * kMaxLevels + l) * 3 + d];
-                const size_t os = off_sup[(v * kMaxLevels + l) * 3 + d];
-                axis_table(I.W[d], (double)I.W[d], I.C[l][d], (AxisEntry*)(hb + oa), (int32_t*)(hb + os));
+                size_t oa = off_ax[(v * kMaxLevels + l) * 3 + d];
+                size_t os = off_sup[(v * kMaxLevels + l) * 3 + d];
+                axis_table_at(I.W[d], (double)I.W[d], I.C[l][d], oa, os, nullptr, &oa, &os);
                 LD.ax[d] = (const AxisEntry*)(base + oa);
                 LD.sup[d] = (const int32_t*)(base + os);
             }
@@ -670,9 +690,9 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         if (need_tables_full) {
             const int* Cf = I.C[L - 1];
             for (int d = 0; d < 3; ++d) {
-                const size_t oa = off_fax[v * 3 + d];
+                size_t oa = off_fax[v * 3 + d], os = 0;
                 std::vector<int32_t> sup(2 * Cf[d]);
-                axis_table(in[v].F[d], (double)I.W[d] * (double)in[v].s, Cf[d], (AxisEntry*)(hb + oa), sup.data());
+                axis_table_at(in[v].F[d], (double)I.W[d] * (double)in[v].s, Cf[d], oa, 0, sup.data(), &oa, &os);
                 D.fax[d] = (const AxisEntry*)(base + oa);
             }
         }
