@@ -110,6 +110,20 @@ HD N4_GLOBAL T* gp(T* p) {
     return (N4_GLOBAL T*)p;
 }
 
+// Four consecutive floats from any 4-byte-aligned address, in one vector
+// load (global loads need only dword alignment).
+HD F4 load_run4(const float* p) {
+#ifdef N4_DEVICE_PASS
+    typedef float v4u __attribute__((ext_vector_type(4), aligned(4)));
+    const v4u x = *reinterpret_cast<const N4_GLOBAL v4u*>(gp(p));
+    return F4{{x[0], x[1], x[2], x[3]}};
+#else
+    F4 r;
+    memcpy(r.v, p, sizeof r.v);
+    return r;
+#endif
+}
+
 // One 16-byte global load of a voxel quad (p 16-byte aligned).
 HD F4 load_quad(const float* p) {
 #ifdef N4_DEVICE_PASS

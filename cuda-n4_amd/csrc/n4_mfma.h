@@ -144,9 +144,9 @@ struct FitMfmaLds {
     N4_LDS AxisEntry* ex;       // the slab's x factors (copied in the set-up round)
 };
 
-// g staging: a wave loads 16 rows x 64 z with consecutive lanes on
-// consecutive z (256 B per load), remaps, and parks the BF16 halves here;
-// the MFMA A operands (8 consecutive z per lane) are then 16-B LDS reads.
+// g staging: a wave loads 16 rows x 64 z (4-voxel runs per lane, see
+// fit_mfma_load), remaps, and parks the BF16 halves here; the MFMA A
+// operands (8 consecutive z per lane) are then 16-B LDS reads.
 constexpr int kGStride = 72;  // halves per staged row (144 B: rows stay 16-B aligned)
 
 HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
@@ -209,17 +209,32 @@ __device__ void fit_mfma_tables_t(const LevelDesc& LD, const FitMfmaLds& S, int 
 }
 
 // The voxel block (16 tile-local rows from y0, 64 z from zc) of one wave:
-// consecutive lanes on consecutive z (256 B per row).
+// lane l takes the 4-voxel run at z = zc + 4 (l & 15) of rows 4i + (l >> 4),
+// i = 0..3 (value 4i + j is row 4i + (l >> 4), z + j): its logU in one
+// 16-byte load (dword-aligned, any row length), its mask bytes one by one;
+// runs cut by the row's end load voxel by voxel.
 __device__ inline void fit_mfma_load(const VolDesc& V, const Slab sl, int y0, int zc, bool den, uint8_t (&ms)[16],
                                      float (&xs)[16]) {
     const int Y = V.W[1], Z = V.W[2];
     const int lane = (int)threadIdx.x & 63;
-    const int z = zc + lane;
-    UNROLL for (int r = 0; r < 16; ++r) {
-        const bool ok = z < Z && y0 + r < sl.ny;
-        const size_t v = ((size_t)sl.x * Y + sl.y0 + (ok ? y0 + r : 0)) * Z + (ok ? z : 0);
-        ms[r] = ok ? gp(V.wmask)[v] : 0;
-        xs[r] = (ok && !den) ? gp(V.logU)[v] : 0.0f;
+    const int zq = zc + 4 * (lane & 15);
+    UNROLL for (int i = 0; i < 4; ++i) {
+        const int r = 4 * i + (lane >> 4);
+        const bool rok = y0 + r < sl.ny && zq < Z;
+        const size_t v = ((size_t)sl.x * Y + sl.y0 + (rok ? y0 + r : 0)) * Z + (rok ? zq : 0);
+        F4 x4 = F4{{0.0f, 0.0f, 0.0f, 0.0f}};
+        if (rok && !den) {
+            if (zq + 4 <= Z) {
+                x4 = load_run4(V.logU + v);
+            } else {
+                UNROLL for (int j = 0; j < 4; ++j)
+                    if (zq + j < Z) x4.v[j] = gp(V.logU)[v + j];
+            }
+        }
+        UNROLL for (int j = 0; j < 4; ++j) {
+            ms[4 * i + j] = (rok && zq + j < Z) ? gp(V.wmask)[v + j] : 0;
+            xs[4 * i + j] = x4.v[j];
+        }
     }
 }
 
@@ -255,8 +270,9 @@ __device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab 
                     bf16_split(gv, h, l);
                 else
                     h = bf16_rn(gv);
-                Gh[r * kGStride + lane] = h;
-                if constexpr (PREC == 2) Gl[r * kGStride + lane] = l;
+                const int slot = (((r >> 2) << 2) + (lane >> 4)) * kGStride + 4 * (lane & 15) + (r & 3);
+                Gh[slot] = h;
+                if constexpr (PREC == 2) Gl[slot] = l;
             }
             __syncthreads();
             UNROLL for (int ks = 0; ks < 2; ++ks) {
