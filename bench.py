@@ -374,6 +374,16 @@ def main():
             step_s.append(time.perf_counter() - t1)
         lib.synchronize()
         dt = time.perf_counter() - t0
+        # calls that plan from scratch: a call with another threshold in
+        # between replaces the handle's cached plan, so each timed call
+        # builds and uploads its plan again (the first call's cost)
+        p_other = n4amd.default_params(precision=prec, **dict(base_kw, threshold=0.002))
+        cold_s = []
+        for _ in range(3):
+            batch_step(p_other)
+            t1 = time.perf_counter()
+            batch_step(p)
+            cold_s.append(time.perf_counter() - t1)
         # kernel durations, measured apart from the timed region with one
         # stream group, direct launches and the default tiles (no concurrent
         # groups sharing the CUs, no graphs; the same launches whatever the
@@ -406,11 +416,12 @@ def main():
              "dominant": dominant, "dom_ms": dom_ms, "dom_n": dom_n, "dom_bytes": dom_bytes, "achieved": achieved,
              "breakdown": {k: round(v[0], 4) for k, v in breakdown.items()},
              "err": sample_errors(its), "schedule": schedule, "autotune_s": round(t_tune, 2),
-             "step_ms_median": 1e3 * float(np.median(step_s))}
+             "step_ms_median": 1e3 * float(np.median(step_s)),
+             "new_plan_ms_median": 1e3 * float(np.median(cold_s))}
         runs[prec] = R
         log(f"[{prec}] batched: {R['ms_per_step']:.3f} ms/step, {R['volumes_per_s']:.1f} volumes/s; dominant "
             f"{dominant}: {dom_ms / max(1, dom_n) * 1e3:.2f} us avg over {dom_n} launches, {achieved:.1f} GB/s; "
-            f"err {R['err']}")
+            f"a call building its plan {R['new_plan_ms_median']:.3f} ms; err {R['err']}")
 
     H = runs[args.precision]
     value = H["volumes_per_s"]
@@ -440,6 +451,9 @@ def main():
                          "same number of steps run again with one stream group, direct launches and the default "
                          "tiles (independent of the timed region's schedule), with events around each of its "
                          "launches",
+        "plan_note": "every timed step has the shapes and parameters of the step before, so it reuses the "
+                     "handle's cached plan (device layout and upload image) and uploads only descriptors and "
+                     "zeroed state; batched.*.ms_per_call_new_plan_median times calls that build their plan",
         "schedule": H["schedule"],
         "schedule_source": {"autotune": "n4_autotune_batch on this workload (groups x tile voxels x graph steps), "
                                         "run before the warmup",
@@ -456,8 +470,8 @@ def main():
     if os.path.exists(ppath):
         with open(ppath) as f:
             pj = json.load(f)
-        tp = pj.get("kernel_timing_pass", {})
-        if tp.get("kernel") == dominant:
+        tp = pj.get("kernel_timing_pass", {}).get(dominant)
+        if tp:
             if tp.get("fetch_bytes") is not None:
                 traffic = round(tp["fetch_bytes"] + (tp.get("write_bytes") or 0.0))
             prof_us = tp.get("rocprof_avg_us")
@@ -479,6 +493,7 @@ def main():
     result["kernel_ms_per_step_one_group"] = H["breakdown"]
     result["batched"] = {q: {"volumes_per_s": round(R["volumes_per_s"], 3), "ms_per_step": round(R["ms_per_step"], 4),
                              "ms_per_step_median": round(R["step_ms_median"], 4),
+                             "ms_per_call_new_plan_median": round(R["new_plan_ms_median"], 4),
                              "mvoxel_iter_per_s": round(R["mvoxel_iter_per_s"], 2),
                              "logbias_err_vs_fp64": R["err"], "kernel_ms_per_step": R["breakdown"],
                              "schedule": R["schedule"], "autotune_s": R["autotune_s"]}
