@@ -2232,8 +2232,8 @@ HD void persist_fit(const VolDesc& V, const LevelDesc& LD, const Slab* my, int n
         const int t = (int)threadIdx.x, wv = t >> 6;
         for (int i = 0; i < nmy; ++i) {
             const Slab sl = my[i];
-            uint8_t ms[16];
-            float xs[16];
+            uint8_t ms[4 * kFitRuns];
+            float xs[4 * kFitRuns];
             fit_mfma_load(V, sl, 16 * wv, 0, den, ms, xs);
             fit_mfma_tables_t<1, 1>(LD, S.M, V.W[2], sl.y0, sl.ny, den ? 2 : 1, false, true, t);
             if (t < (int)(sizeof(AxisEntry) / 4))
@@ -2567,9 +2567,11 @@ __global__ __launch_bounds__(kBlock) void k_flat(const VolDesc* __restrict__ vol
 }
 
 // The one-tile instance (up to 16 control points along y and z) is held to
-// 96 VGPRs, so 5 workgroups fit a CU (its LDS limit) instead of 4.
+// 80 VGPRs in BF16 (6 waves per SIMD: 6 workgroups per CU, as many as its
+// LDS allows for tiles of up to 72 x 72 voxels) and to 96 in split BF16
+// (at 80 it spills).
 template <int PREC, int MT, int NT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT * NT == 1 ? 5 : 1))) void k_fit_mfma(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level, int it,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT * NT == 1 ? (PREC == 1 ? 6 : 5) : 1))) void k_fit_mfma(const VolDesc* __restrict__ vols, const Slab* __restrict__ slabs, int level, int it,
                                                      int mode, RunParams rp) {
     HIP_DYNAMIC_SHARED(double, smem_);
     body_fit_mfma<PREC, MT, NT>(vols, slabs, level, it, mode, rp, blockIdx.x, SMEM());

@@ -29,6 +29,7 @@ namespace n4 {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // float -> BF16 bits, round to nearest even (finite inputs).
@@ -144,10 +145,13 @@ struct FitMfmaLds {
     N4_LDS AxisEntry* ex;       // the slab's x factors (copied in the set-up round)
 };
 
-// g staging: a wave loads 16 rows x 64 z (4-voxel runs per lane, see
-// fit_mfma_load), remaps, and parks the BF16 halves here; the MFMA A
-// operands (8 consecutive z per lane) are then 16-B LDS reads.
+// g staging: a wave loads 16 rows x one z chunk of at most 64 (4-voxel runs
+// spread over the lanes, see fit_mfma_load), remaps, and parks the BF16
+// halves here; the MFMA A operands (8 consecutive z per lane) are then 16-B
+// LDS reads.
 constexpr int kGStride = 72;  // halves per staged row (144 B: rows stay 16-B aligned)
+constexpr int kFitRuns = 4;   // 4-voxel runs per lane: 16 rows x 64 z over 64 lanes
+HD inline int fit_chunk_width(int Z, int zc) { return Z - zc > 64 ? 64 : Z - zc; }
 
 HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
     Carve c{base, 0};
@@ -208,20 +212,35 @@ __device__ void fit_mfma_tables_t(const LevelDesc& LD, const FitMfmaLds& S, int 
     }
 }
 
-// The voxel block (16 tile-local rows from y0, 64 z from zc) of one wave:
-// lane l takes the 4-voxel run at z = zc + 4 (l & 15) of rows 4i + (l >> 4),
-// i = 0..3 (value 4i + j is row 4i + (l >> 4), z + j): its logU in one
-// 16-byte load (dword-aligned, any row length), its mask bytes one by one;
-// runs cut by the row's end load voxel by voxel.
-__device__ inline void fit_mfma_load(const VolDesc& V, const Slab sl, int y0, int zc, bool den, uint8_t (&ms)[16],
-                                     float (&xs)[16]) {
+// The voxel block of one wave: 16 tile-local rows from y0, the z chunk
+// from zc (fit_chunk_width).  Its 16 x nq 4-voxel runs (nq runs per row
+// of the chunk) are spread over the lanes: lane l takes runs q = l + 64 i
+// (i < kFitRuns, q < 16 nq), run q being row q / nq at z = zc + 4 (q % nq),
+// so a lane issues ceil(nq / 4) loads of each kind whatever the row length.
+// Value 4i + j is z + j of run i: its logU from one 16-byte load
+// (dword-aligned), its mask bytes one by one; runs cut by the row's end
+// load voxel by voxel; runs outside the tile hold zeros.
+struct FitRun {
+    int row, col;  // block row, chunk-local z of the run's first voxel
+    bool live;     // q < 16 nq
+};
+__device__ inline FitRun fit_run(int nq, float rq, int i) {
+    const int q = ((int)threadIdx.x & 63) + 64 * i;
+    // q / nq for q < 256, nq <= 16: the quotient's fractional part is at
+    // least 1/32 from an integer, far above the float error
+    const int row = (int)(((float)q + 0.5f) * rq);
+    return FitRun{row, 4 * (q - row * nq), q < 16 * nq};
+}
+__device__ inline void fit_mfma_load(const VolDesc& V, const Slab sl, int y0, int zc, bool den,
+                                     uint8_t (&ms)[4 * kFitRuns], float (&xs)[4 * kFitRuns]) {
     const int Y = V.W[1], Z = V.W[2];
-    const int lane = (int)threadIdx.x & 63;
-    const int zq = zc + 4 * (lane & 15);
-    UNROLL for (int i = 0; i < 4; ++i) {
-        const int r = 4 * i + (lane >> 4);
-        const bool rok = y0 + r < sl.ny && zq < Z;
-        const size_t v = ((size_t)sl.x * Y + sl.y0 + (rok ? y0 + r : 0)) * Z + (rok ? zq : 0);
+    const int nq = (fit_chunk_width(Z, zc) + 3) >> 2;
+    const float rq = 1.0f / (float)nq;
+    UNROLL for (int i = 0; i < kFitRuns; ++i) {
+        const FitRun u = fit_run(nq, rq, i);
+        const int zq = zc + u.col;  // < Z for a live run
+        const bool rok = u.live && y0 + u.row < sl.ny;
+        const size_t v = ((size_t)sl.x * Y + sl.y0 + (rok ? y0 + u.row : 0)) * Z + (rok ? zq : 0);
         F4 x4 = F4{{0.0f, 0.0f, 0.0f, 0.0f}};
         if (rok && !den) {
             if (zq + 4 <= Z) {
@@ -244,8 +263,8 @@ __device__ inline void fit_mfma_load(const VolDesc& V, const Slab sl, int y0, in
 // stage calls).
 template <int PREC, int MT, int NT>
 __device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab sl, const FitMfmaLds& S, int gmode,
-                              bool den, bool scatter, bool degenerate, const Remap& rm, uint8_t (&ms)[16],
-                              float (&xs)[16]) {
+                              bool den, bool scatter, bool degenerate, const Remap& rm,
+                              uint8_t (&ms)[4 * kFitRuns], float (&xs)[4 * kFitRuns]) {
     const int Z = V.W[2];
     const int Cy = LD.C[1], Cz = LD.C[2];
     const int x = sl.x;
@@ -256,28 +275,55 @@ __device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab 
 
     N4_LDS unsigned short* Gh = S.Gh + wv * 16 * kGStride;
     N4_LDS unsigned short* Gl = (PREC == 2) ? S.Gl + wv * 16 * kGStride : nullptr;
+    // The staged block's columns past a chunk's runs (up to its last K
+    // step) meet zero rows of Pz but are read: they start as zeros (the
+    // LDS may hold anything) and later hold only staged finite values.
+    // A wave's LDS accesses run in order, so its own block needs no barrier.
+    {
+        constexpr int kQ = 16 * kGStride / 8;  // 16-byte words of the wave's block
+        UNROLL for (int k = lane; k < kQ; k += 64) {
+            reinterpret_cast<N4_LDS u16x8*>(Gh)[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            if constexpr (PREC == 2) reinterpret_cast<N4_LDS u16x8*>(Gl)[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        }
+    }
     // every wave runs the same trip counts (barriers inside)
     for (int rb = 0; rb < Yt; rb += 64) {
         const int y0 = rb + 16 * wv;  // this wave's 16 tile-local rows (past Yt: zeros)
         f32x4 t1[NT];
         UNROLL for (int n = 0; n < NT; ++n) t1[n] = f32x4{0, 0, 0, 0};
-        for (int zc = 0; zc < Z; zc += 64) {
+        for (int zc = 0; zc < Z;) {
+            const int zw = fit_chunk_width(Z, zc);
             if (rb > 0 || zc > 0) fit_mfma_load(V, sl, y0, zc, den, ms, xs);  // (the first block is already in registers)
-            UNROLL for (int r = 0; r < 16; ++r) {
-                const float gv = fit_input(gmode, ms[r], xs[r], degenerate, rm);
-                unsigned short h, l = 0;
-                if constexpr (PREC == 2)
-                    bf16_split(gv, h, l);
-                else
-                    h = bf16_rn(gv);
-                const int slot = (((r >> 2) << 2) + (lane >> 4)) * kGStride + 4 * (lane & 15) + (r & 3);
-                Gh[slot] = h;
-                if constexpr (PREC == 2) Gl[slot] = l;
+            const int nq = (zw + 3) >> 2;
+            const float rq = 1.0f / (float)nq;
+            UNROLL for (int i = 0; i < kFitRuns; ++i) {
+                const FitRun u = fit_run(nq, rq, i);
+                if (u.live) {
+                    // the run's 4 halves in one 8-byte store (rows are 144 B, runs 8 B aligned)
+                    uint32_t h2[2], l2[2] = {0u, 0u};
+                    UNROLL for (int j = 0; j < 4; j += 2) {
+                        unsigned short h0, h1, l0 = 0, l1 = 0;
+                        const float g0 = fit_input(gmode, ms[4 * i + j], xs[4 * i + j], degenerate, rm);
+                        const float g1 = fit_input(gmode, ms[4 * i + j + 1], xs[4 * i + j + 1], degenerate, rm);
+                        if constexpr (PREC == 2) {
+                            bf16_split(g0, h0, l0);
+                            bf16_split(g1, h1, l1);
+                        } else {
+                            h0 = bf16_rn(g0);
+                            h1 = bf16_rn(g1);
+                        }
+                        h2[j >> 1] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+                        l2[j >> 1] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+                    }
+                    const int slot = u.row * kGStride + u.col;
+                    *reinterpret_cast<N4_LDS u32x2*>(Gh + slot) = u32x2{h2[0], h2[1]};
+                    if constexpr (PREC == 2) *reinterpret_cast<N4_LDS u32x2*>(Gl + slot) = u32x2{l2[0], l2[1]};
+                }
             }
             __syncthreads();
             UNROLL for (int ks = 0; ks < 2; ++ks) {
                 const int z0 = zc + 32 * ks;
-                if (z0 < Z) {
+                if (32 * ks < zw) {
                     const u16x8 ah = *reinterpret_cast<const N4_LDS u16x8*>(Gh + r16 * kGStride + 32 * ks + 8 * g4);
                     const u16x8 al =
                         (PREC == 2) ? *reinterpret_cast<const N4_LDS u16x8*>(Gl + r16 * kGStride + 32 * ks + 8 * g4) : ah;
@@ -289,6 +335,7 @@ __device__ void fit_mfma_tile(const VolDesc& V, const LevelDesc& LD, const Slab 
                 }
             }
             __syncthreads();
+            zc += zw;
         }
         if (y0 >= Yt) continue;  // no rows: nothing to add (and no Py rows)
         // t1[n]: lane holds T1[y0 + 4*g4 + i][16n + r16] = the K=16 operand (k = y)
@@ -361,8 +408,8 @@ __device__ void body_fit_mfma(const VolDesc* vols, const Slab* slabs, int level,
     const Remap rm = make_remap(S.E, bmin, slope, nb);
     // the first block of voxels (row batch 0, z chunk 0) is loaded before
     // the table set-up, so the two global round trips overlap
-    uint8_t ms[16];
-    float xs[16];
+    uint8_t ms[4 * kFitRuns];
+    float xs[4 * kFitRuns];
     fit_mfma_load(V, sl, 16 * wv, 0, den, ms, xs);
     if (num)
         for (int i = t; i < nb; i += kBlock) S.E[i] = (float)V.etab[i];
