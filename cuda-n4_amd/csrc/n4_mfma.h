@@ -151,7 +151,7 @@ struct FitMfmaLds {
 // LDS reads.
 constexpr int kGStride = 72;  // halves per staged row (144 B: rows stay 16-B aligned)
 constexpr int kFitRuns = 4;   // 4-voxel runs per lane: 16 rows x 64 z over 64 lanes
-HD inline int fit_chunk_width(int Z, int zc) { return Z - zc > 64 ? 64 : Z - zc; }
+HD int fit_chunk_width(int Z, int zc) { return Z - zc > 64 ? 64 : Z - zc; }
 
 HD size_t carve_fit_mfma(N4_LDS unsigned char* base, int nbins, int Y, int Z, int MT, int NT, int prec, FitMfmaLds* o) {
     Carve c{base, 0};
