@@ -793,12 +793,14 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
     const VolDesc& V = vols[sl.vol];
     VolState* st = rp.states + sl.vol;
     const int nb = rp.nbins;
-    Carve cv{smem, 0};
-    N4_LDS double* red = cv.take<double>(4 * kBlock);
+    // The partial reduction's scratch and the bins share the block's LDS
+    // (the bins are cleared once the reduction's results are in registers),
+    // so the kernel's LDS is the larger of the two and 8 workgroups fit a CU.
+    N4_LDS double* red = Carve{smem, 0}.take<double>(4 * kBlock);
     // kHistLanes lane-strided copies of the bins (odd stride: the copies of
     // one bin fall in different LDS banks), merged before the global flush
     const int hs = nb | 1;
-    N4_LDS unsigned long long* h = cv.take<unsigned long long>((size_t)kHistLanes * hs);
+    N4_LDS unsigned long long* h = Carve{smem, 0}.take<unsigned long long>((size_t)kHistLanes * hs);
     const int mode = step < 0 ? kModeIter : st->mode;
     const int lvl = step < 0 ? 0 : st->lvl, it = step < 0 ? 0 : st->it;
     if (mode == kModeFinished) {
@@ -856,6 +858,7 @@ HD void body_hist(const VolDesc* vols, const Slab* slabs, int step, RunParams rp
         BLOCK_END
     }
     if (kind == kStepIter && !degenerate) {
+        BLOCK_SYNC();  // every thread has read the reduction's results (the bins overlay them)
         hist_zero(h, hs);
         BLOCK_SYNC();
         hist_tile(V, sl, mn, slope, nb, h, hs, dbg_idx);
@@ -2462,7 +2465,7 @@ static int cp_tiles(int c) { return (c + 15) / 16; }
 size_t lds_prep() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_level() { return align16(4 * kBlock * sizeof(double)); }
 size_t lds_hist(int nbins) {
-    return align16(4 * kBlock * sizeof(double)) + align16((size_t)kHistLanes * (nbins | 1) * sizeof(uint64_t));
+    return std::max(align16(4 * kBlock * sizeof(double)), align16((size_t)kHistLanes * (nbins | 1) * sizeof(uint64_t)));
 }
 size_t lds_wiener(int P) { return align16(4 * (size_t)P * sizeof(D2)); }
 size_t lds_lattice() { return align16(kBlock * sizeof(double)); }
