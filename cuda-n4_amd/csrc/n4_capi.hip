@@ -96,7 +96,19 @@ struct n4_handle {
     int device = 0;
     void* arena = nullptr;
     size_t arena_bytes = 0;
-    std::vector<unsigned char> staging;  // host image of the upload region
+    // Host image of a plan's upload region (stage): pinned host memory
+    // (direct DMA) for ordinary calls, the pageable vector for the
+    // allocation-free entry point (it allocates no pinned memory) and the
+    // host emulation.  stage_ev marks the last upload from pinned memory:
+    // the image is rewritten only after it.
+    unsigned char* stage = nullptr;
+    std::vector<unsigned char> staging;
+    unsigned char* stage_pinned = nullptr;
+    size_t stage_pinned_cap = 0;
+#ifndef N4_EMULATE
+    hipEvent_t stage_ev = nullptr;
+    bool stage_ev_pending = false;
+#endif
     uint64_t* pinned_flags = nullptr;    // flag_cap progress words (pinned host)
     uint64_t* dev_flags = nullptr;       // the same memory as seen by the device
     int flag_cap = 0;
@@ -131,7 +143,7 @@ struct n4_handle {
     // The last correction call's plan, reused by a call with the same key
     // (volume shapes, shrink factors, mask kinds, parameters, tiling, team
     // sizes, progress words, kernel timing) while no other plan has been
-    // built since: the staging buffer then still holds its upload image, and
+    // built since: the stage then still holds its upload image, and
     // only the caller's buffer addresses in the descriptors change.
     std::vector<unsigned char> plan_key;
     std::unique_ptr<Plan> plan_cached;
@@ -325,7 +337,7 @@ struct Plan {
     int nlevels = 0;
     RunParams rp;
     std::vector<VolDesc> desc;  // host copies (pointers into the arena)
-    size_t upload_bytes = 0;    // [0, upload_bytes) of the arena comes from staging
+    size_t upload_bytes = 0;    // [0, upload_bytes) of the arena comes from the stage
     size_t total_bytes = 0;
     size_t off_desc = 0, off_wslab = 0, off_fslab = 0, off_pslab = 0, off_state = 0, off_tw = 0;
     int n_wslab = 0, n_fslab = 0, n_pslab = 0;
@@ -443,7 +455,62 @@ static int rows_for(int rows, int cols, int target) {
     return r < rows ? r : rows;
 }
 
-// Builds the plan and fills h->staging with the upload region.  Pointers in
+// The last upload from the stage has completed.
+static void stage_wait(n4_handle* h) {
+#ifndef N4_EMULATE
+    if (h->stage_ev_pending) {
+        (void)hipEventSynchronize(h->stage_ev);
+        h->stage_ev_pending = false;
+    }
+#else
+    (void)h;
+#endif
+}
+
+// A zeroed stage of n bytes: pinned when allowed (growing the pinned buffer
+// if needed; should pinned memory be refused, the pageable vector serves).
+static void stage_reset(n4_handle* h, size_t n, bool pinned_ok) {
+    stage_wait(h);
+#ifndef N4_EMULATE
+    if (pinned_ok && h->stage_pinned_cap < n) {
+        if (h->stage_pinned) (void)hipHostFree(h->stage_pinned);
+        h->stage_pinned = nullptr;
+        h->stage_pinned_cap = 0;
+        void* q = nullptr;
+        if (hipHostMalloc(&q, n, hipHostMallocDefault) == hipSuccess) {
+            h->stage_pinned = (unsigned char*)q;
+            h->stage_pinned_cap = n;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    if (pinned_ok && h->stage_pinned) {
+        memset(h->stage_pinned, 0, n);
+        h->stage = h->stage_pinned;
+        return;
+    }
+#else
+    (void)pinned_ok;
+#endif
+    h->staging.assign(n, 0);
+    h->stage = h->staging.data();
+}
+
+// Enqueues a copy of n bytes at offset off of the stage to dst on stream st
+// and, for a pinned stage, marks it (stage_wait).
+static n4_status stage_upload(n4_handle* h, void* dst, size_t off, size_t n, void* st) {
+    H2D(dst, h->stage + off, n, st);
+#ifndef N4_EMULATE
+    if (h->stage == h->stage_pinned) {
+        if (!h->stage_ev) N4_CHECK_HIP(hipEventCreateWithFlags(&h->stage_ev, hipEventDisableTiming));
+        N4_CHECK_HIP(hipEventRecord(h->stage_ev, (hipStream_t)st));
+        h->stage_ev_pending = true;
+    }
+#endif
+    return N4_OK;
+}
+
+// Builds the plan and fills h->stage with the upload region.  Pointers in
 // the descriptors are arena-relative until `rebase` adds the arena base.
 // tiled: cut working slabs into row tiles for the step kernels (the
 // correction); stage calls keep whole slabs (their partials are per slab).
@@ -461,7 +528,7 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     const int nv = (int)in.size();
     const int L = (int)p->number_of_fitting_levels;
     h->last.valid = false;  // the plan reuses the memory of the last run's records
-    h->plan_cached.reset();  // the staging buffer and the arena are rewritten
+    h->plan_cached.reset();  // the stage and the arena are rewritten
     P->nvol = nv;
     P->nlevels = L;
     P->rp = make_run_params(p);
@@ -587,8 +654,8 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     P->rp.descs = (const VolDesc*)(base + P->off_desc);
 
     // ---- host image of the upload region
-    h->staging.assign(P->upload_bytes, 0);
-    unsigned char* hb = h->staging.data();
+    stage_reset(h, P->upload_bytes, !noalloc);
+    unsigned char* hb = h->stage;
     P->desc.assign(nv, VolDesc());
     Slab* ws = (Slab*)(hb + P->off_wslab);
     Slab* fs = (Slab*)(hb + P->off_fslab);
@@ -1207,7 +1274,7 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
     if (!noalloc) pkey = correction_plan_key(h, in, p, sch.tile_vox, teams_path ? &teams : nullptr);
     const bool plan_reused = !noalloc && h->plan_cached && pkey == h->plan_key;
     if (plan_reused) {
-        // the plan of the last call, whose upload image is still in staging:
+        // the plan of the last call, whose upload image is still in the stage:
         // only the caller's buffers change
         P = *h->plan_cached;
         h->last.valid = false;  // its run records are reused
@@ -1220,7 +1287,8 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
             D.out_logbias = in[v].out_logbias;
             D.out_lattice = in[v].out_lattice;
         }
-        memcpy(h->staging.data() + P.off_desc, P.desc.data(), sizeof(VolDesc) * nv);
+        stage_wait(h);
+        memcpy(h->stage + P.off_desc, P.desc.data(), sizeof(VolDesc) * nv);
     } else {
         s = build_plan(h, in, p, &P, true, true, sch.tile_vox, teams_path ? &teams : nullptr, noalloc);
         if (s != N4_OK) return s;
@@ -1235,11 +1303,12 @@ static n4_status run_batch(n4_handle* h, const std::vector<VolIn>& in_user, cons
         // the arena still holds the rest of the upload region (slab lists,
         // twiddles, axis tables; no kernel writes them): descriptors, and the
         // volume states back to their initial (zero) image
-        H2D(P.base + P.off_desc, h->staging.data() + P.off_desc, sizeof(VolDesc) * nv, stream);
-        H2D(P.base + P.off_state, h->staging.data() + P.off_state, sizeof(VolState) * nv, stream);
+        s = stage_upload(h, P.base + P.off_desc, P.off_desc, sizeof(VolDesc) * nv, stream);
+        if (s == N4_OK) s = stage_upload(h, P.base + P.off_state, P.off_state, sizeof(VolState) * nv, stream);
     } else {
-        H2D(P.base, h->staging.data(), P.upload_bytes, stream);
+        s = stage_upload(h, P.base, 0, P.upload_bytes, stream);
     }
+    if (s != N4_OK) return s;
     VolDesc* dd = dev_desc(h, P);
     Slab* wsl = dev_wslab(h, P);
     Slab* fsl = dev_fslab(h, P);
@@ -1621,6 +1690,8 @@ n4_status n4_destroy(n4_handle* h) {
     dev_free(h->arena);
 #ifndef N4_EMULATE
     if (h->pinned_flags) (void)hipHostFree(h->pinned_flags);
+    if (h->stage_pinned) (void)hipHostFree(h->stage_pinned);
+    if (h->stage_ev) (void)hipEventDestroy(h->stage_ev);
     if (h->events_init) {
         (void)hipEventDestroy(h->fork_ev);
         for (int g = 0; g < kMaxGroups; ++g) (void)hipEventDestroy(h->join_ev[g]);
@@ -1888,8 +1959,9 @@ static n4_status flat_plan(n4_handle* h, uint64_t n, int nbins, Plan* P, const f
         D.logU = const_cast<float*>(x);
         D.wmask = const_cast<uint8_t*>(m);
     }
-    memcpy(h->staging.data() + P->off_desc, &D, sizeof D);
-    H2D(P->base, h->staging.data(), P->upload_bytes, nullptr);
+    memcpy(h->stage + P->off_desc, &D, sizeof D);
+    s = stage_upload(h, P->base, 0, P->upload_bytes, nullptr);
+    if (s != N4_OK) return s;
     if (hist) {
         const size_t nw = (size_t)nslab * kChunkVox;
         D2D(D.logU, x, sizeof(float) * n, nullptr);
@@ -1991,7 +2063,8 @@ n4_status n4_fit_bspline3d(n4_handle* h, const float* d_field, const uint8_t* d_
     s = check_lds(P, true, false);
     if (s != N4_OK) return s;
     VolDesc& D = P.desc[0];
-    H2D(P.base, h->staging.data(), P.upload_bytes, nullptr);
+    s = stage_upload(h, P.base, 0, P.upload_bytes, nullptr);
+    if (s != N4_OK) return s;
     // the fit kernel takes 16-byte loads of aligned voxel quads: it reads
     // the plan's own padded copies of the field and the mask
     const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
@@ -2029,8 +2102,9 @@ n4_status n4_evaluate_bspline3d(n4_handle* h, const float* d_lattice, const uint
     if (s != N4_OK) return s;
     VolDesc& D = P.desc[0];
     D.latf = const_cast<float*>(d_lattice);
-    memcpy(h->staging.data() + P.off_desc, &D, sizeof D);
-    H2D(P.base, h->staging.data(), P.upload_bytes, nullptr);
+    memcpy(h->stage + P.off_desc, &D, sizeof D);
+    s = stage_upload(h, P.base, 0, P.upload_bytes, nullptr);
+    if (s != N4_OK) return s;
     LAUNCH(launch_recon(dev_desc(h, P), dev_fslab(h, P), P.n_fslab, 0, P.rp, P.lds_recon, nullptr));
     SYNC(nullptr);
     return N4_OK;
@@ -2089,7 +2163,8 @@ n4_status n4_upsample_lattice3d(n4_handle* h, const float* d_lattice, const uint
     Plan P;
     s = build_plan(h, std::vector<VolIn>{v}, &p, &P, false);
     if (s != N4_OK) return s;
-    H2D(P.base, h->staging.data(), P.upload_bytes, nullptr);
+    s = stage_upload(h, P.base, 0, P.upload_bytes, nullptr);
+    if (s != N4_OK) return s;
     const VolDesc& D = P.desc[0];
     const size_t L0 = (size_t)D.lev[0].L, L1 = (size_t)D.lev[1].L;
     std::vector<float> lf(L0);
