@@ -20,6 +20,7 @@
 #include <atomic>
 #include <chrono>
 #include <map>
+#include <thread>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -467,8 +468,10 @@ static void stage_wait(n4_handle* h) {
 #endif
 }
 
-// A zeroed stage of n bytes: pinned when allowed (growing the pinned buffer
-// if needed; should pinned memory be refused, the pageable vector serves).
+// A stage of n bytes, pinned when allowed (growing the pinned buffer if
+// needed; should pinned memory be refused, the pageable vector serves).
+// Its content is left as it is: build_plan writes every byte a kernel reads
+// and zeroes the volume states.
 static void stage_reset(n4_handle* h, size_t n, bool pinned_ok) {
     stage_wait(h);
 #ifndef N4_EMULATE
@@ -485,14 +488,13 @@ static void stage_reset(n4_handle* h, size_t n, bool pinned_ok) {
         }
     }
     if (pinned_ok && h->stage_pinned) {
-        memset(h->stage_pinned, 0, n);
         h->stage = h->stage_pinned;
         return;
     }
 #else
     (void)pinned_ok;
 #endif
-    h->staging.assign(n, 0);
+    if (h->staging.size() < n) h->staging.resize(n);
     h->stage = h->staging.data();
 }
 
@@ -578,17 +580,43 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     if (teams)
         for (int v = 0; v < nv; ++v) P->n_work += (*teams)[v];
     P->off_work = lay.take(sizeof(PWork) * std::max(P->n_work, 1));
-    // tables
+    // axis tables: identical ones (same positions, domain and control
+    // points) are laid out and computed once per plan and shared by every
+    // volume and level using them; working-grid tables carry their support
+    // ranges, full-resolution tables do not
+    struct AxisJob {
+        int npos;
+        double domain;
+        int ncpt;
+        size_t oa, os;  // entries; support ranges (working-grid tables)
+        bool full;
+    };
+    std::vector<AxisJob> ax_jobs;
+    std::map<std::tuple<bool, int, double, int>, size_t> ax_index;
+    auto axis_slot = [&](bool full, int npos, double domain, int ncpt) -> const AxisJob& {
+        const auto key = std::make_tuple(full, npos, domain, ncpt);
+        auto it = ax_index.find(key);
+        if (it == ax_index.end()) {
+            AxisJob j{npos, domain, ncpt, lay.take(sizeof(AxisEntry) * npos, 64),
+                      full ? 0 : lay.take(sizeof(int32_t) * 2 * ncpt, 16), full};
+            it = ax_index.emplace(key, ax_jobs.size()).first;
+            ax_jobs.push_back(j);
+        }
+        return ax_jobs[it->second];
+    };
     std::vector<size_t> off_ax(nv * kMaxLevels * 3), off_sup(nv * kMaxLevels * 3), off_fax(nv * 3);
     for (int v = 0; v < nv; ++v)
         for (int l = 0; l < L; ++l)
             for (int d = 0; d < 3; ++d) {
-                off_ax[(v * kMaxLevels + l) * 3 + d] = lay.take(sizeof(AxisEntry) * vi[v].W[d], 64);
-                off_sup[(v * kMaxLevels + l) * 3 + d] = lay.take(sizeof(int32_t) * 2 * vi[v].C[l][d], 16);
+                const AxisJob& j = axis_slot(false, vi[v].W[d], (double)vi[v].W[d], vi[v].C[l][d]);
+                off_ax[(v * kMaxLevels + l) * 3 + d] = j.oa;
+                off_sup[(v * kMaxLevels + l) * 3 + d] = j.os;
             }
     if (need_tables_full)
         for (int v = 0; v < nv; ++v)
-            for (int d = 0; d < 3; ++d) off_fax[v * 3 + d] = lay.take(sizeof(AxisEntry) * in[v].F[d], 64);
+            for (int d = 0; d < 3; ++d)
+                off_fax[v * 3 + d] =
+                    axis_slot(true, in[v].F[d], (double)vi[v].W[d] * (double)in[v].s, vi[v].C[L - 1][d]).oa;
     P->upload_bytes = lay.take(0);
 
     // per-volume working buffers
@@ -669,25 +697,32 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         }
         P->rp.twiddle = (const D2*)(base + P->off_tw);
     }
-    // identical axis tables (same positions, domain and control points) are
-    // computed once per plan and shared by every volume and level using them
-    // (working-grid tables, with their support ranges, and full-resolution
-    // tables, without, are kept apart)
-    std::map<std::tuple<bool, int, double, int>, std::pair<size_t, size_t>> ax_memo;
-    auto axis_table_at = [&](int npos, double domain, int ncpt, size_t oa, size_t os, int32_t* sup_tmp,
-                             size_t* oa_used, size_t* os_used) {
-        const auto key = std::make_tuple(sup_tmp != nullptr, npos, domain, ncpt);
-        const auto it = ax_memo.find(key);
-        if (it != ax_memo.end()) {
-            *oa_used = it->second.first;
-            *os_used = it->second.second;
-            return;
-        }
-        axis_table(npos, domain, ncpt, (AxisEntry*)(hb + oa), sup_tmp ? sup_tmp : (int32_t*)(hb + os));
-        ax_memo.emplace(key, std::make_pair(oa, os));
-        *oa_used = oa;
-        *os_used = os;
-    };
+    // the volume states start at zero (the rest of the image is written
+    // below wherever a kernel reads it)
+    memset(hb + P->off_state, 0, sizeof(VolState) * nv);
+    // the distinct axis tables, on up to 8 host threads when there are many
+    {
+        const int nj = (int)ax_jobs.size();
+        auto run_jobs = [&](int k0, int step) {
+            std::vector<int32_t> sup_tmp;
+            for (int k = k0; k < nj; k += step) {
+                const AxisJob& j = ax_jobs[k];
+                int32_t* sup = (int32_t*)(hb + j.os);
+                if (j.full) {
+                    sup_tmp.assign(2 * (size_t)j.ncpt, 0);
+                    sup = sup_tmp.data();
+                }
+                axis_table(j.npos, j.domain, j.ncpt, (AxisEntry*)(hb + j.oa), sup);
+            }
+        };
+        size_t entries = 0;
+        for (const AxisJob& j : ax_jobs) entries += (size_t)j.npos;
+        const int nt = entries >= 8192 ? std::min(8, std::max(1, (int)std::thread::hardware_concurrency())) : 1;
+        std::vector<std::thread> pool;
+        for (int k = 1; k < nt; ++k) pool.emplace_back(run_jobs, k, nt);
+        run_jobs(0, nt);
+        for (std::thread& th : pool) th.join();
+    }
     for (int v = 0; v < nv; ++v) {
         const VInfo& I = vi[v];
         VolDesc D;
@@ -745,24 +780,14 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
             LevelDesc& LD = D.lev[l];
             for (int d = 0; d < 3; ++d) {
                 LD.C[d] = I.C[l][d];
-                size_t oa = off_ax[(v * kMaxLevels + l) * 3 + d];
-                size_t os = off_sup[(v * kMaxLevels + l) * 3 + d];
-                axis_table_at(I.W[d], (double)I.W[d], I.C[l][d], oa, os, nullptr, &oa, &os);
-                LD.ax[d] = (const AxisEntry*)(base + oa);
-                LD.sup[d] = (const int32_t*)(base + os);
+                LD.ax[d] = (const AxisEntry*)(base + off_ax[(v * kMaxLevels + l) * 3 + d]);
+                LD.sup[d] = (const int32_t*)(base + off_sup[(v * kMaxLevels + l) * 3 + d]);
             }
             LD.L = LD.C[0] * LD.C[1] * LD.C[2];
             LD.mCz = magic_div((uint32_t)LD.C[2]);
         }
-        if (need_tables_full) {
-            const int* Cf = I.C[L - 1];
-            for (int d = 0; d < 3; ++d) {
-                size_t oa = off_fax[v * 3 + d], os = 0;
-                std::vector<int32_t> sup(2 * Cf[d]);
-                axis_table_at(in[v].F[d], (double)I.W[d] * (double)in[v].s, Cf[d], oa, 0, sup.data(), &oa, &os);
-                D.fax[d] = (const AxisEntry*)(base + oa);
-            }
-        }
+        if (need_tables_full)
+            for (int d = 0; d < 3; ++d) D.fax[d] = (const AxisEntry*)(base + off_fax[v * 3 + d]);
         P->desc[v] = D;
         if (teams) {
             PWork* wk = (PWork*)(hb + P->off_work);
