@@ -701,11 +701,14 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
     // below wherever a kernel reads it)
     memset(hb + P->off_state, 0, sizeof(VolState) * nv);
     // the distinct axis tables, on up to 8 host threads when there are many
+    // (each thread takes the next table; the calling thread works too, so
+    // every table is computed even if no other thread could be started)
     {
         const int nj = (int)ax_jobs.size();
-        auto run_jobs = [&](int k0, int step) {
+        std::atomic<int> next{0};
+        auto run_jobs = [&]() {
             std::vector<int32_t> sup_tmp;
-            for (int k = k0; k < nj; k += step) {
+            for (int k = next.fetch_add(1); k < nj; k = next.fetch_add(1)) {
                 const AxisJob& j = ax_jobs[k];
                 int32_t* sup = (int32_t*)(hb + j.os);
                 if (j.full) {
@@ -719,8 +722,14 @@ static n4_status build_plan(n4_handle* h, const std::vector<VolIn>& in, const n4
         for (const AxisJob& j : ax_jobs) entries += (size_t)j.npos;
         const int nt = entries >= 8192 ? std::min(8, std::max(1, (int)std::thread::hardware_concurrency())) : 1;
         std::vector<std::thread> pool;
-        for (int k = 1; k < nt; ++k) pool.emplace_back(run_jobs, k, nt);
-        run_jobs(0, nt);
+        for (int k = 1; k < nt; ++k) {
+            try {
+                pool.emplace_back(run_jobs);
+            } catch (...) {
+                break;
+            }
+        }
+        run_jobs();
         for (std::thread& th : pool) th.join();
     }
     for (int v = 0; v < nv; ++v) {
