@@ -300,12 +300,34 @@ HD void reduce_partials(const double* part, int nslab, N4_LDS double* red) {
             r3 = fmax(r3, hi[u].b);
         }
     }
+#ifdef N4_DEVICE_PASS
+    // tree_reduce4's order (wave butterflies, then the waves in order) on
+    // values still in registers: no LDS round trip and two barriers fewer
+    double v[4] = {r0, r1, r2, r3};
+    const int ops[4] = {0, 0, 1, 2};
+    UNROLL for (int sh = 32; sh > 0; sh >>= 1)
+        UNROLL for (int k = 0; k < 4; ++k) v[k] = red_apply(ops[k], v[k], __shfl_xor(v[k], sh));
+    if ((t & 63) == 0)
+        UNROLL for (int k = 0; k < 4; ++k) red[k * kBlock + (t >> 6)] = v[k];
+    __syncthreads();
+    if (t == 0) {
+        UNROLL for (int k = 0; k < 4; ++k) {
+            double r = red[k * kBlock];
+            for (int i = 1; i < kBlock / 64; ++i) r = red_apply(ops[k], r, red[k * kBlock + i]);
+            red[k * kBlock] = r;
+        }
+    }
+    __syncthreads();
+#else
     red[t] = r0;
     red[kBlock + t] = r1;
     red[2 * kBlock + t] = r2;
     red[3 * kBlock + t] = r3;
+#endif
     BLOCK_END
+#ifndef N4_DEVICE_PASS
     tree_reduce4(red, 0, 0, 1, 2);
+#endif
 }
 
 // Coefficient of variation of exp(old - new) over n voxels from
