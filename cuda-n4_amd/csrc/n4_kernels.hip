@@ -275,6 +275,33 @@ HD void tree_reduce4(N4_LDS double* red, int o0, int o1, int o2, int o3) {
 #endif
 }
 
+#ifdef N4_DEVICE_PASS
+// tree_reduce4's results (red[k * kBlock], its order, so the same bits)
+// from each thread's four values in registers: wave butterflies, the waves'
+// results through LDS, combined in wave order by thread 0; no LDS round
+// trip for the threads' values.  The wave results overwrite slots of red:
+// no thread may still read red when the block enters.
+__device__ void reduce4_regs(double a, double b, double c, double d, N4_LDS double* red, int o0, int o1, int o2,
+                             int o3) {
+    const int t = (int)threadIdx.x;
+    const int ops[4] = {o0, o1, o2, o3};
+    double v[4] = {a, b, c, d};
+    UNROLL for (int sh = 32; sh > 0; sh >>= 1)
+        UNROLL for (int k = 0; k < 4; ++k) v[k] = red_apply(ops[k], v[k], __shfl_xor(v[k], sh));
+    if ((t & 63) == 0)
+        UNROLL for (int k = 0; k < 4; ++k) red[k * kBlock + (t >> 6)] = v[k];
+    __syncthreads();
+    if (t == 0) {
+        UNROLL for (int k = 0; k < 4; ++k) {
+            double r = red[k * kBlock];
+            for (int i = 1; i < kBlock / 64; ++i) r = red_apply(ops[k], r, red[k * kBlock + i]);
+            red[k * kBlock] = r;
+        }
+    }
+    __syncthreads();
+}
+#endif
+
 // Reduces n partial quadruples part[i] = (s0, s1, min, max).  Thread t
 // folds partials t, t + kBlock, ... in that order; their loads are issued
 // eight at a time (two 16-byte loads per partial) before they are folded.
@@ -301,23 +328,7 @@ HD void reduce_partials(const double* part, int nslab, N4_LDS double* red) {
         }
     }
 #ifdef N4_DEVICE_PASS
-    // tree_reduce4's order (wave butterflies, then the waves in order) on
-    // values still in registers: no LDS round trip and two barriers fewer
-    double v[4] = {r0, r1, r2, r3};
-    const int ops[4] = {0, 0, 1, 2};
-    UNROLL for (int sh = 32; sh > 0; sh >>= 1)
-        UNROLL for (int k = 0; k < 4; ++k) v[k] = red_apply(ops[k], v[k], __shfl_xor(v[k], sh));
-    if ((t & 63) == 0)
-        UNROLL for (int k = 0; k < 4; ++k) red[k * kBlock + (t >> 6)] = v[k];
-    __syncthreads();
-    if (t == 0) {
-        UNROLL for (int k = 0; k < 4; ++k) {
-            double r = red[k * kBlock];
-            for (int i = 1; i < kBlock / 64; ++i) r = red_apply(ops[k], r, red[k * kBlock + i]);
-            red[k * kBlock] = r;
-        }
-    }
-    __syncthreads();
+    reduce4_regs(r0, r1, r2, r3, red, 0, 0, 1, 2);  // red is not read before
 #else
     red[t] = r0;
     red[kBlock + t] = r1;
